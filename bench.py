@@ -1,0 +1,160 @@
+"""Benchmark: temporal set inversion on one MI355X.
+
+One step = one frame of the plain per-frame path on the 64-capsule workload (seeded
+smooth-min union of 64 capsules translating linearly in t under a twist warp of angle
+2 t z; localsubdiv 9 = 512^3 cells; 1024x1024 frame; delta 0.01; Lipschitz correction on):
+set inversion (tsi_set_inversion) + trace of the frame (tsi_trace) + root re-enqueue,
+at t_k = k/60.  The first W frames are untimed warm-up (frame 0 evaluates every cell);
+the next K frames are timed with the device synchronised on both sides.
+
+Prints one JSON line.  `value` = interval cell-evals per second over the timed frames;
+`roofline` reports the tape-evaluation kernels against the FP32 VALU peak; `cpu_baseline`
+times the oracle (the C restatement of the reference) on a bounded sample of the same tape.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X FP32 vector issue: 256 CU x 4 SIMD x 16 lanes x 4 (wave64 over 4 cycles) ... expressed as
+# lane-results: 157.3 TFLOP/s counts an FMA as 2 and assumes packed dual-FP32 issue, i.e.
+# 78.6 T f32 results/s; without packed issue 39.3 T/s.
+PEAK_VALU_TOPS = 78.6
+PEAK_VALU_UNPACKED_TOPS = 39.3
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    # plain method on one MI355X
+    "capsules64_L9_1024": dict(scene="capsules", seed=1, localsubdiv=9, width=1024, height=1024,
+                               lipschitz_corrected=True, swept_volume=False, point_sample=True),
+    # plain method, large (1024^3, swept, point-sampled)
+    "csg256_L10_1920x1080": dict(scene="csg", seed=2, localsubdiv=10, width=1920, height=1080,
+                                 lipschitz_corrected=False, swept_volume=True, point_sample=True),
+    # configs[0] at GPU scale: 8 moving spheres
+    "spheres8_L6_256": dict(scene="spheres", seed=0, localsubdiv=6, width=256, height=256,
+                            lipschitz_corrected=False, swept_volume=False, point_sample=True),
+}
+
+
+def cpu_baseline(tape, w, budget_s: float):
+    """Oracle (single-threaded C restatement) on a bounded sample: the same tape and
+    parameters at a reduced depth, frames t_k = k/60 until the time budget is spent."""
+    import oracle
+
+    L = max(3, w["localsubdiv"] - 3)
+    fr = oracle.Frame(tape, localsubdiv=L, swept_volume=w["swept_volume"],
+                      lipschitz_corrected=w["lipschitz_corrected"], point_sample=w["point_sample"],
+                      voxel_capacity=1 << 22)
+    cells = 0
+    frames = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and frames < 120:
+        st = fr.set_inversion(float(np.float32(frames / 60.0)))
+        cells += st.cell_evals
+        frames += 1
+        fr.enqueue_root()
+    dt = time.perf_counter() - t0
+    fr.close()
+    return {"value": cells / dt, "unit": "cell-evals/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, same tape at localsubdiv {L}, {frames} frames t=k/60, {cells} cell-evals, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="capsules64_L9_1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by octree "
+              f"subtree, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import temporalsetinversion_amd as T
+    from temporalsetinversion_amd import scenes
+
+    w = WORKLOADS[args.workload]
+    ll = scenes.SCENES[w["scene"]](w["seed"])
+    tape = T.tape_from_ll(ll)
+    st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
+                 point_sample=w["point_sample"], swept_volume=w["swept_volume"],
+                 lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27)
+    st.reset()
+    frame = 0
+
+    def step():
+        nonlocal frame
+        st.update(float(np.float32(frame / 60.0)))
+        r = st.set_inversion()
+        st.render(pixels=False)
+        frame += 1
+        return r
+
+    for _ in range(args.warmup):
+        step()
+    tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+        s = r.stats
+        tot["cells"] += s["cell_evals"]
+        tot["rev"] += s["rev_evals"]
+        tot["fwd_ops"] += s["fwd_ops"]
+        tot["rev_ops"] += s["rev_ops"]
+        tot["gpu_ms"] += s["gpu_ms"]
+        tot["eval_ms"] += s["eval_ms"]
+        tot["launches"] += s["eval_launches"]
+    wall = time.perf_counter() - t0
+    st.close()
+
+    ops = tot["fwd_ops"] + tot["rev_ops"]
+    achieved = ops / (tot["eval_ms"] * 1e-3) / 1e12 if tot["eval_ms"] > 0 else 0.0
+    out = {
+        "metric": "interval cell-evals/s",
+        "value": tot["cells"] / wall,
+        "unit": "cell-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "tape_instructions": int(len(tape)), "localsubdiv": w["localsubdiv"],
+                   "frame": f"{w['width']}x{w['height']}", "delta": 0.01,
+                   "lipschitz_corrected": w["lipschitz_corrected"], "swept_volume": w["swept_volume"],
+                   "point_sample": w["point_sample"], "times": "t_k = k/60"},
+        "set_inversion_ms_per_frame": tot["gpu_ms"] / args.steps,
+        "cell_evals_per_frame": tot["cells"] / args.steps,
+        "rev_evals_per_frame": tot["rev"] / args.steps,
+        "frames_per_s": args.steps / wall,
+        "roofline": {"bound": "valu", "kernel": "k_eval (tape evaluation, forward + reverse)",
+                     "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "T f32-ops/s",
+                     "frac": achieved / PEAK_VALU_TOPS, "traffic": None,
+                     "peak_unpacked": PEAK_VALU_UNPACKED_TOPS,
+                     "eval_ms_per_frame": tot["eval_ms"] / args.steps,
+                     "ops_per_frame": ops / args.steps},
+    }
+    if not args.no_cpu:
+        import oracle
+
+        out["cpu_baseline"] = cpu_baseline(oracle.parse_ll(ll), w, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * tsi.h -- C ABI of the MI355X temporal set-inversion library (libtsi.so).
+ *
+ * This is the drop-in boundary for the path the reference dispatches through wgpu in
+ * src/main.rs.  Each entry point names the reference code it replaces (paths relative to
+ * the reference checkout).  Plain pointers and sizes only; no C++ exceptions, no aborts
+ * cross this boundary; every call returns TSI_OK (0) or a negative TSI_E_* code.
+ * A tsi_ctx is not thread-safe: one context per host thread.  Calls are blocking, as the
+ * reference's per-frame timestamp map is (src/main.rs:799-806).
+ */
+#ifndef TSI_H
+#define TSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TSI_OK = 0,
+    TSI_E_OOM = -1,              /* device or host allocation failed */
+    TSI_E_QUEUE_FULL = -2,       /* a frontier list overflowed (the reference silently drops, compute.wgsl:3256) */
+    TSI_E_POOL_EXHAUSTED = -3,   /* voxel pool full (the reference does not check, compute.wgsl:3440) */
+    TSI_E_TAPE_TOO_LONG = -4,    /* > 4096 instructions (the reference truncates, src/ir.rs:358) */
+    TSI_E_HIP = -5,              /* a HIP runtime call failed */
+    TSI_E_ARG = -6,              /* invalid argument / malformed tape */
+    TSI_E_PARSE = -7,            /* .ll text not understood (src/ir.rs would panic) */
+    TSI_E_NO_TAPE = -8,          /* no tape uploaded yet */
+    TSI_E_UNSUPPORTED = -9,      /* tape needs more live values than the register file holds */
+    TSI_E_BUDGET = -10           /* served form: request exceeds the HBM admission budget */
+};
+
+/* src/ir.rs:88-97 GPUInstruction, 32 bytes: opcode_uses = (op | side<<14) << 16 | uses,
+ * larg_rarg = larg << 16 | rarg, side = 0 reg/reg, 1 reg/lit, 2 lit/reg, 3 lit/lit. */
+typedef struct {
+    uint32_t index_;
+    uint32_t opcode_uses;
+    uint32_t larg_rarg;
+    float literal;
+    float val[2];
+    float sens[2];
+} tsi_gpu_instruction;
+
+/* src/uniforms.rs:8-17, 96 bytes.  view = inverse view matrix, column-major (camera.rs:33). */
+typedef struct {
+    float view[16];
+    uint32_t resolution[2];
+    float time;
+    float lasttime;
+    float rendermode;
+    float pad[3];
+} tsi_uniforms;
+
+/* compute.wgsl:3494-3511 PARAMETERS as runtime values, plus pool sizing. */
+typedef struct {
+    int32_t localsubdiv;          /* octree depth L: (2^L)^3 leaves over the root box */
+    float delta;                  /* temporal error bound */
+    int32_t point_sample;         /* evaluate leaf cells at their centre */
+    int32_t swept_volume;
+    int32_t lipschitz_corrected;
+    int32_t fastenabled;
+    float fast_multiplier;
+    float swept_window;           /* 1/60 in compute.wgsl:3605 */
+    uint64_t voxel_capacity;      /* nodes in the pool (0 = default) */
+    uint64_t queue_capacity;      /* entries per frontier list (0 = voxel_capacity / 8) */
+    uint32_t flags;               /* TSI_FLAG_* */
+    uint32_t reserved;
+} tsi_params;
+
+enum {
+    TSI_FLAG_NO_PRUNE = 1u        /* evaluate the full tape in every cell (the reference's behaviour) */
+};
+
+/* Per-frame statistics.  fwd_ops/rev_ops count algorithmic f32 interval operations. */
+typedef struct {
+    double gpu_ms;                /* device time of the set-inversion dispatches */
+    double eval_ms;               /* of which: tape-evaluation kernels */
+    uint64_t cell_evals;          /* child cells that passed the evaluation guard (compute.wgsl:3629) */
+    uint64_t rev_evals;           /* of those, cells that ran the reverse pass (compute.wgsl:3780) */
+    uint64_t fwd_ops;
+    uint64_t rev_ops;
+    uint64_t parents;             /* non-dealloc work items (parents subdivided) */
+    uint64_t deallocs;            /* node blocks freed by the dealloc cascade */
+    uint64_t enqueued;            /* cells enqueued for subdivision */
+    uint64_t allocated_nodes;     /* live pool nodes after the frame */
+    uint64_t eval_launches;
+    uint64_t pruned_instr;        /* sum over evaluated cells of instructions skipped by pruning */
+} tsi_stats;
+
+/* Canonical node record (see tsi_read_nodes). */
+typedef struct {
+    uint32_t depth, i, j, k;
+    float f, t, time_err, fast;
+    uint32_t has_children;
+    float dx, dy, dz;
+} tsi_node;
+
+typedef struct tsi_ctx tsi_ctx;
+
+/* Defaults of compute.wgsl:3494-3511 with the given depth. */
+void tsi_default_params(tsi_params *p, int32_t localsubdiv);
+
+/* State::new buffers and pipelines (src/main.rs:97-496): creates the device pool, frontier
+ * lists and stream on the current HIP device. */
+int tsi_create(const tsi_params *params, tsi_ctx **out);
+void tsi_destroy(tsi_ctx *ctx);
+
+/* First-frame tape upload (src/main.rs:742-746): validates and lowers n GPUInstruction
+ * records (n <= 4096 else TSI_E_TAPE_TOO_LONG).  The host copy is kept for tsi_read_tape. */
+int tsi_upload_tape(tsi_ctx *ctx, const tsi_gpu_instruction *tape, uint32_t n);
+
+/* first_eval block (src/main.rs:730-766): clear pool and queues, root node = root_box
+ * (x.lo, y.lo, z.lo, x.hi, y.hi, z.hi), enqueue the root. */
+int tsi_reset(tsi_ctx *ctx, const float root_box[6]);
+
+/* State::update uniform write (src/main.rs:503). */
+int tsi_set_uniforms(tsi_ctx *ctx, const tsi_uniforms *u);
+
+/* One `main` dispatch (src/main.rs:776-814, compute.wgsl:3513-4056) at uniforms.time.
+ * Blocking.  *gpu_ms (may be NULL) receives the device time, like State::compute's return. */
+int tsi_set_inversion(tsi_ctx *ctx, double *gpu_ms, tsi_stats *stats);
+
+/* rt_main (src/main.rs:649-656, compute.wgsl:4650-4688): traces the frame into the
+ * library's device pixel buffer and copies it to rgba_out (W*H*4 floats, row 0 on top)
+ * and depth_out (W*H floats, -1 = miss); either may be NULL. */
+int tsi_trace(tsi_ctx *ctx, float *rgba_out, float *depth_out);
+
+/* Device pointers of the frame written by the last tsi_trace (valid until the next
+ * tsi_trace or tsi_destroy), for callers that keep the frame on the GPU. */
+int tsi_frame_device(tsi_ctx *ctx, const float **rgba_dev, const float **depth_dev);
+
+/* buf_main (src/main.rs:683-690, compute.wgsl:4690-4707): re-enqueue the root so the
+ * next tsi_set_inversion re-walks the octree. */
+int tsi_enqueue_root(tsi_ctx *ctx);
+
+/* Debug/parity export (cf. the tape/voxel dumps of src/main.rs:822-876): every node
+ * reachable from the root through nodes with children, as tsi_node records keyed by
+ * (depth, i, j, k).  Writes min(cap, count) records; *count receives the total. */
+int tsi_read_nodes(tsi_ctx *ctx, tsi_node *out, uint64_t cap, uint64_t *count);
+
+/* The uploaded tape as 32-byte records (src/main.rs:822-876 tape dump). */
+int tsi_read_tape(tsi_ctx *ctx, tsi_gpu_instruction *out, uint32_t cap, uint32_t *n);
+
+/* src/ir.rs IR::new + parseLLVMIR + to_gpu: builds the tape from .ll text holding
+ * @_Z1fffff.  time is written into instruction 3 (ARGW).  *n receives the full
+ * instruction count (also when > cap or > 4096). */
+int tsi_tape_from_ll(const char *text, size_t len, float time, tsi_gpu_instruction *out, uint32_t cap,
+                     uint32_t *n);
+
+/* z = res.y / tan(radians(90)/2) of compute.wgsl:4592 for a frame of the given height,
+ * as the library computes it. */
+float tsi_camera_zdist(uint32_t height);
+
+/* Text of a TSI_E_* code. */
+const char *tsi_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+"""MI355X temporal set inversion: host mirror of the reference's interface over libtsi.so.
+
+The reference's host (src/main.rs, src/ir.rs) builds a tape from clang `.ll` text and
+drives three compute entry points per frame through wgpu.  This package keeps that
+interface -- `IR` for the tape builder, `State` with `update` / `compute` / `render`
+for the frame protocol -- and calls the HIP kernels through the C ABI of
+include/tsi.h.  There is no CPU fallback: every call goes to the device library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import GpuInstruction, Params, Stats, TsiError, Uniforms, check, lib
+
+__all__ = ["IR", "State", "Params", "Stats", "TsiError", "look_at_inverse", "diagonal_camera", "tape_from_ll",
+           "INSTR_DTYPE", "NODE_DTYPE", "OPCODES"]
+
+INSTR_DTYPE = np.dtype([("index_", "<u4"), ("opcode_uses", "<u4"), ("larg_rarg", "<u4"), ("literal", "<f4"),
+                        ("val", "<f4", 2), ("sens", "<f4", 2)])
+NODE_DTYPE = np.dtype([("depth", "<u4"), ("i", "<u4"), ("j", "<u4"), ("k", "<u4"), ("f", "<f4"), ("t", "<f4"),
+                       ("time_err", "<f4"), ("fast", "<f4"), ("has_children", "<u4"), ("dx", "<f4"),
+                       ("dy", "<f4"), ("dz", "<f4")])
+
+# src/ir.rs:8-43
+OPCODES = {0: "NOP", 1: "ARGX", 2: "ARGY", 3: "ARGZ", 4: "ARGW", 5: "ADD", 6: "SUB", 7: "MUL", 8: "DIV", 9: "EXP",
+           10: "LOG", 11: "SQR", 12: "SQRT", 13: "POW", 14: "SIN", 15: "COS", 16: "NEG", 17: "MIN", 18: "MAX",
+           19: "ABS", 20: "CUP", 21: "CAP", 22: "BR", 23: "PHI", 24: "TRUNC", 25: "FLOOR", 26: "CEIL", 27: "SIGN",
+           28: "RET", 29: "MOD", 30: "ATAN2"}
+
+
+def tape_from_ll(text: str | bytes, time: float = 0.0) -> np.ndarray:
+    """src/ir.rs IR::new: `.ll` text -> 32-byte GPUInstruction records (INSTR_DTYPE)."""
+    b = text.encode() if isinstance(text, str) else text
+    n = C.c_uint32(0)
+    rc = lib().tsi_tape_from_ll(b, len(b), time, None, 0, C.byref(n))
+    if rc not in (0, -4):
+        check(rc, "tsi_tape_from_ll")
+    cap = min(n.value, 4096)
+    out = (GpuInstruction * max(cap, 1))()
+    rc = lib().tsi_tape_from_ll(b, len(b), time, out, cap, C.byref(n))
+    check(rc, "tsi_tape_from_ll")
+    return np.frombuffer(bytes(out), dtype=INSTR_DTYPE, count=cap).copy()
+
+
+class IR:
+    """Mirror of src/ir.rs `IR`: the tape of one scene."""
+
+    def __init__(self, ll_text: str | bytes, time: float = 0.0):
+        self.gpu_instructions = tape_from_ll(ll_text, time)
+
+    def __len__(self):
+        return len(self.gpu_instructions)
+
+    def ops(self) -> np.ndarray:
+        return (self.gpu_instructions["opcode_uses"] >> 16) & 0x3FFF
+
+    def describe(self, j: int) -> str:
+        g = self.gpu_instructions[j]
+        op = int(g["opcode_uses"] >> 16) & 0x3FFF
+        return (f"index = {int(g['index_'])}, uses = {int(g['opcode_uses']) & 0xFFFF}, op = {OPCODES.get(op, op)}, "
+                f"larg = {int(g['larg_rarg']) >> 16}, rarg = {int(g['larg_rarg']) & 0xFFFF}, "
+                f"rarg_literal = {float(g['literal'])}")
+
+
+def look_at_inverse(eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """Camera-to-world matrix (camera.rs:33-35: inverse of cgmath look_at_dir), column-major
+    float32[16] as uniforms.view holds it: columns (side, up, -forward, eye)."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.zeros((4, 4), np.float64)
+    m[0, :3], m[1, :3], m[2, :3], m[3, :3] = s, u, -f, eye
+    m[3, 3] = 1.0
+    return m.astype(np.float32).reshape(16)
+
+
+def diagonal_camera(distance: float = 12.0) -> np.ndarray:
+    """Eye on the (1,1,1) diagonal at `distance` from the origin, looking at the origin."""
+    e = distance / np.sqrt(3.0)
+    return look_at_inverse((e, e, e))
+
+
+def default_params(localsubdiv: int = 8) -> Params:
+    p = Params()
+    lib().tsi_default_params(C.byref(p), localsubdiv)
+    return p
+
+
+@dataclass
+class FrameResult:
+    gpu_ms: float
+    stats: dict
+
+
+class State:
+    """Mirror of src/main.rs `State`'s frame protocol over the HIP library.
+
+    update(time, view)  -> State::update (uniform write, main.rs:498-504)
+    compute()           -> State::compute (first-frame seeding + `main`, main.rs:724-814);
+                           returns the device milliseconds, like the reference
+    render()            -> State::render's rt_main (649-656) then buf_main (683-690)
+    """
+
+    def __init__(self, tape, width: int = 256, height: int = 256, localsubdiv: int = 8, delta: float = 0.01,
+                 point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
+                 fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
+                 voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune: bool = True):
+        if isinstance(tape, IR):
+            tape = tape.gpu_instructions
+        elif isinstance(tape, (str, bytes)):
+            tape = tape_from_ll(tape)
+        self.tape = np.ascontiguousarray(tape, dtype=INSTR_DTYPE)
+        p = default_params(localsubdiv)
+        p.delta = delta
+        p.point_sample = int(point_sample)
+        p.swept_volume = int(swept_volume)
+        p.lipschitz_corrected = int(lipschitz_corrected)
+        p.fastenabled = int(fastenabled)
+        p.fast_multiplier = fast_multiplier
+        p.swept_window = np.float32(swept_window)
+        p.voxel_capacity = voxel_capacity
+        p.queue_capacity = queue_capacity
+        p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
+        self.params = p
+        self._ctx = C.c_void_p()
+        check(lib().tsi_create(C.byref(p), C.byref(self._ctx)), "tsi_create")
+        buf = (GpuInstruction * len(self.tape)).from_buffer_copy(self.tape.tobytes())
+        check(lib().tsi_upload_tape(self._ctx, buf, len(self.tape)), "tsi_upload_tape")
+        self.uniforms = Uniforms()
+        self.uniforms.resolution[0] = width
+        self.uniforms.resolution[1] = height
+        v = diagonal_camera()
+        for i in range(16):
+            self.uniforms.view[i] = float(v[i])
+        self.first_eval = True
+        self.width, self.height = width, height
+
+    # -- frame protocol --------------------------------------------------------
+    def update(self, time: float, view=None):
+        u = self.uniforms
+        u.lasttime = u.time
+        u.time = float(np.float32(time))
+        if view is not None:
+            for i, x in enumerate(np.asarray(view, np.float32).reshape(16)):
+                u.view[i] = float(x)
+        check(lib().tsi_set_uniforms(self._ctx, C.byref(u)), "tsi_set_uniforms")
+
+    def reset(self):
+        v = self.tape["val"]
+        box = (C.c_float * 6)(float(v[0, 0]), float(v[1, 0]), float(v[2, 0]), float(v[0, 1]), float(v[1, 1]),
+                              float(v[2, 1]))
+        check(lib().tsi_reset(self._ctx, box), "tsi_reset")
+
+    def compute(self) -> float:
+        if self.first_eval:
+            self.reset()
+            self.first_eval = False
+        return self.set_inversion().gpu_ms
+
+    def set_inversion(self) -> FrameResult:
+        ms = C.c_double(0.0)
+        st = Stats()
+        check(lib().tsi_set_inversion(self._ctx, C.byref(ms), C.byref(st)), "tsi_set_inversion")
+        return FrameResult(ms.value, st.as_dict())
+
+    def trace(self, pixels: bool = True):
+        rgba = np.zeros((self.height, self.width, 4), np.float32) if pixels else None
+        depth = np.zeros((self.height, self.width), np.float32) if pixels else None
+        check(lib().tsi_trace(self._ctx, rgba.ctypes.data if pixels else None,
+                              depth.ctypes.data if pixels else None), "tsi_trace")
+        return rgba, depth
+
+    def enqueue_root(self):
+        check(lib().tsi_enqueue_root(self._ctx), "tsi_enqueue_root")
+
+    def render(self, pixels: bool = True):
+        out = self.trace(pixels)
+        self.enqueue_root()
+        return out
+
+    # -- parity / debug ----------------------------------------------------------
+    def read_nodes(self) -> np.ndarray:
+        n = C.c_uint64(0)
+        check(lib().tsi_read_nodes(self._ctx, None, 0, C.byref(n)), "tsi_read_nodes")
+        out = np.zeros(n.value, NODE_DTYPE)
+        check(lib().tsi_read_nodes(self._ctx, out.ctypes.data, n.value, C.byref(n)), "tsi_read_nodes")
+        return out
+
+    def read_tape(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        check(lib().tsi_read_tape(self._ctx, None, 0, C.byref(n)), "tsi_read_tape")
+        out = (GpuInstruction * n.value)()
+        check(lib().tsi_read_tape(self._ctx, out, n.value, C.byref(n)), "tsi_read_tape")
+        return np.frombuffer(bytes(out), dtype=INSTR_DTYPE).copy()
+
+    def close(self):
+        if self._ctx:
+            lib().tsi_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def camera_zdist(height: int) -> float:
+    return float(lib().tsi_camera_zdist(height))

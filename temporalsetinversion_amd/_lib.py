@@ -1,0 +1,105 @@
+"""ctypes binding of libtsi.so (include/tsi.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails loudly
+when libtsi.so has not been built (run __graft_entry__.build() or
+`python temporalsetinversion_amd/build.py`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("TSI_LIB", os.path.join(HERE, "_build", "libtsi.so"))
+
+TSI_OK = 0
+ERRORS = {
+    -1: "TSI_E_OOM", -2: "TSI_E_QUEUE_FULL", -3: "TSI_E_POOL_EXHAUSTED", -4: "TSI_E_TAPE_TOO_LONG",
+    -5: "TSI_E_HIP", -6: "TSI_E_ARG", -7: "TSI_E_PARSE", -8: "TSI_E_NO_TAPE", -9: "TSI_E_UNSUPPORTED",
+    -10: "TSI_E_BUDGET",
+}
+TSI_FLAG_NO_PRUNE = 1
+
+
+class TsiError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        super().__init__(f"{what}: {ERRORS.get(code, code)} ({code})")
+
+
+class GpuInstruction(C.Structure):
+    """ir.rs:88-97 GPUInstruction (32 B)."""
+    _fields_ = [("index_", C.c_uint32), ("opcode_uses", C.c_uint32), ("larg_rarg", C.c_uint32),
+                ("literal", C.c_float), ("val", C.c_float * 2), ("sens", C.c_float * 2)]
+
+
+class Uniforms(C.Structure):
+    """uniforms.rs:8-17 (96 B)."""
+    _fields_ = [("view", C.c_float * 16), ("resolution", C.c_uint32 * 2), ("time", C.c_float),
+                ("lasttime", C.c_float), ("rendermode", C.c_float), ("pad", C.c_float * 3)]
+
+
+class Params(C.Structure):
+    """compute.wgsl:3494-3511 PARAMETERS + pool sizing."""
+    _fields_ = [("localsubdiv", C.c_int32), ("delta", C.c_float), ("point_sample", C.c_int32),
+                ("swept_volume", C.c_int32), ("lipschitz_corrected", C.c_int32), ("fastenabled", C.c_int32),
+                ("fast_multiplier", C.c_float), ("swept_window", C.c_float), ("voxel_capacity", C.c_uint64),
+                ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("gpu_ms", C.c_double), ("eval_ms", C.c_double), ("cell_evals", C.c_uint64),
+                ("rev_evals", C.c_uint64), ("fwd_ops", C.c_uint64), ("rev_ops", C.c_uint64),
+                ("parents", C.c_uint64), ("deallocs", C.c_uint64), ("enqueued", C.c_uint64),
+                ("allocated_nodes", C.c_uint64), ("eval_launches", C.c_uint64), ("pruned_instr", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Node(C.Structure):
+    _fields_ = [("depth", C.c_uint32), ("i", C.c_uint32), ("j", C.c_uint32), ("k", C.c_uint32),
+                ("f", C.c_float), ("t", C.c_float), ("time_err", C.c_float), ("fast", C.c_float),
+                ("has_children", C.c_uint32), ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float)]
+
+
+# name -> (restype, argtypes); the list every test checks against include/tsi.h
+SIGNATURES = {
+    "tsi_default_params": (None, [C.POINTER(Params), C.c_int32]),
+    "tsi_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "tsi_destroy": (None, [C.c_void_p]),
+    "tsi_upload_tape": (C.c_int, [C.c_void_p, C.POINTER(GpuInstruction), C.c_uint32]),
+    "tsi_reset": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "tsi_set_uniforms": (C.c_int, [C.c_void_p, C.POINTER(Uniforms)]),
+    "tsi_set_inversion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(Stats)]),
+    "tsi_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tsi_frame_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "tsi_enqueue_root": (C.c_int, [C.c_void_p]),
+    "tsi_read_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "tsi_read_tape": (C.c_int, [C.c_void_p, C.POINTER(GpuInstruction), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "tsi_tape_from_ll": (C.c_int, [C.c_char_p, C.c_size_t, C.c_float, C.POINTER(GpuInstruction), C.c_uint32,
+                                   C.POINTER(C.c_uint32)]),
+    "tsi_camera_zdist": (C.c_float, [C.c_uint32]),
+    "tsi_strerror": (C.c_char_p, [C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != TSI_OK:
+        raise TsiError(rc, what)

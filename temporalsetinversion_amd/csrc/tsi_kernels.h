@@ -1,0 +1,74 @@
+// tsi_kernels.h -- device data layout and launcher interface of the set-inversion kernels.
+//
+// Node pool (structure of arrays in HBM, 64 B per node, 16-byte aligned vector accesses):
+//   st[n]   = {f, t, time_err, meta}   meta bit31 = fast, bits 0-30 = child block + 1 (0: leaf)
+//   blo[n]  = {x.lo, y.lo, z.lo, 0}    box
+//   bhi[n]  = {x.hi, y.hi, z.hi, 0}
+//   grad[n] = {dx, dy, dz, 0}          sensitivity midpoints (compute.wgsl:3967-3969)
+// Children of a node are one block of 8 consecutive nodes (child index x + 2y + 4z, the
+// numbering of compute.wgsl:3576 and 4295).  Node 0 is the root; block 0 holds only it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tsi_tape.h"
+
+namespace tsi {
+
+constexpr int kMaxLevels = 24;
+enum : uint32_t { kErrQueue = 1u, kErrPool = 2u };
+enum { kStatCells = 0, kStatRevCells, kStatParents, kStatEnqueued, kStatRevOps, kStatDeallocs, kStatLive,
+       kNumStats = 8 };
+
+struct Counters {
+    uint32_t list_count[kMaxLevels + 2][2];   // frontier sizes per level: [0] forward-only, [1] reverse
+    uint32_t dlist_count[kMaxLevels + 2];     // dealloc cascade list sizes
+    uint32_t alloc_tickets;                   // blocks handed out this frame
+    uint32_t free_top;                        // free-stack entries at frame start
+    uint32_t bump;                            // blocks ever taken from the bump region
+    uint32_t pending;                         // blocks freed this frame
+    uint32_t frame_error;                     // kErr* raised this frame
+    uint32_t error;                           // sticky kErr* since reset
+    uint32_t root_pending;                    // buf_main has enqueued the root
+    uint32_t pad;
+    uint64_t stats[kNumStats];
+};
+
+struct KArgs {
+    float4 *st, *blo, *bhi, *grad;
+    uint32_t *lists[2][2];      // [level parity][forward-only / reverse]
+    uint32_t *dlists[2];
+    uint32_t *pending_free;
+    uint32_t *free_stack;
+    Counters *cnt;
+    float2 *stack;              // reverse-pass operand stack, stack_len * 64 float2 per wavefront
+    const FwdIns *fprog;
+    const RevIns *rprog;
+    uint32_t nfwd, nrev, stack_len, out_slot;
+    uint32_t arg_slot[4];
+    uint32_t nblocks, list_cap, dlist_cap;
+    float epsilon, delta, fast_multiplier, swept_window;
+    int32_t swept, point_sample, lipschitz, fastenabled;
+};
+
+struct TraceArgs {
+    float view[16];
+    uint32_t width, height;
+    float time, zdist;
+    float4 *rgba;
+    float *depth;
+    unsigned long long *visits;
+};
+
+struct FrameLaunch {
+    int levels;
+    float time;
+    uint32_t grid_fwd, grid_rev, grid_misc;
+};
+
+int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1);
+int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
+int launch_enqueue_root(const KArgs &a, hipStream_t s);
+int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
+
+}  // namespace tsi

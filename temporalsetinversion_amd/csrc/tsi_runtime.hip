@@ -1,0 +1,401 @@
+// tsi_runtime.hip -- the C ABI of include/tsi.h: context, device buffers, frame protocol.
+//
+// Replaces the wgpu half of src/main.rs: State::new's buffers (208-303), State::compute's
+// first-frame seeding and `main` dispatch (724-814) and the compute half of State::render
+// (649-656 rt_main, 683-690 buf_main).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tsi.h"
+#include "tsi_kernels.h"
+#include "tsi_tape.h"
+
+namespace tsi {
+int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd, std::vector<RevIns> &rev,
+          Lowered &L);
+uint32_t fwd_cost(uint32_t op, uint32_t side);
+}  // namespace tsi
+
+using namespace tsi;
+
+struct tsi_ctx {
+    tsi_params p{};
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t nodes = 0;
+    uint32_t nblocks = 0, list_cap = 0, dlist_cap = 0;
+    float4 *st = nullptr, *blo = nullptr, *bhi = nullptr, *grad = nullptr;
+    uint32_t *lists[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t *dlists[2] = {nullptr, nullptr};
+    uint32_t *pending = nullptr, *free_stack = nullptr;
+    Counters *cnt = nullptr;
+    float2 *stack = nullptr;
+    size_t stack_bytes = 0;
+    FwdIns *fprog = nullptr;
+    RevIns *rprog = nullptr;
+    float4 *rgba = nullptr;
+    float *depth = nullptr;
+    size_t frame_px = 0;
+    unsigned long long *visits = nullptr;
+    std::vector<tsi_gpu_instruction> tape;
+    Lowered low{};
+    bool have_tape = false;
+    tsi_uniforms u{};
+    uint32_t grid_fwd = 0, grid_rev = 0, grid_misc = 0;
+    uint64_t last_visits = 0;
+};
+
+namespace {
+
+#define HIP_OK(x) do { if ((x) != hipSuccess) return TSI_E_HIP; } while (0)
+
+template <class T>
+int dmalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return TSI_E_OOM;
+    }
+    return TSI_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+KArgs make_args(const tsi_ctx *c) {
+    KArgs a{};
+    a.st = c->st; a.blo = c->blo; a.bhi = c->bhi; a.grad = c->grad;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) a.lists[i][j] = c->lists[i][j];
+    a.dlists[0] = c->dlists[0];
+    a.dlists[1] = c->dlists[1];
+    a.pending_free = c->pending;
+    a.free_stack = c->free_stack;
+    a.cnt = c->cnt;
+    a.stack = c->stack;
+    a.fprog = c->fprog;
+    a.rprog = c->rprog;
+    a.nfwd = c->low.nfwd;
+    a.nrev = c->low.nrev;
+    a.stack_len = c->low.stack;
+    a.out_slot = c->low.out_slot;
+    for (int k = 0; k < 4; ++k) a.arg_slot[k] = c->low.arg_slot[k];
+    a.nblocks = c->nblocks;
+    a.list_cap = c->list_cap;
+    a.dlist_cap = c->dlist_cap;
+    a.epsilon = 10.0f * float(std::ldexp(1.0, -c->p.localsubdiv));   // compute.wgsl:3521
+    a.delta = c->p.delta;
+    a.fast_multiplier = c->p.fast_multiplier;
+    a.swept_window = c->p.swept_window;
+    a.swept = c->p.swept_volume;
+    a.point_sample = c->p.point_sample;
+    a.lipschitz = c->p.lipschitz_corrected;
+    a.fastenabled = c->p.fastenabled;
+    return a;
+}
+
+void release(tsi_ctx *c) {
+    dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
+    for (auto &l : c->lists) for (auto &x : l) dfree(x);
+    dfree(c->dlists[0]); dfree(c->dlists[1]);
+    dfree(c->pending); dfree(c->free_stack); dfree(c->cnt); dfree(c->stack);
+    dfree(c->fprog); dfree(c->rprog); dfree(c->rgba); dfree(c->depth); dfree(c->visits);
+    for (auto &e : c->ev) if (e) { hipEventDestroy(e); e = nullptr; }
+    if (c->stream) { hipStreamDestroy(c->stream); c->stream = nullptr; }
+}
+
+}  // namespace
+
+extern "C" {
+
+void tsi_default_params(tsi_params *p, int32_t localsubdiv) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->localsubdiv = localsubdiv;          // compute.wgsl:3502 uses 8
+    p->delta = 0.01f;
+    p->point_sample = 1;
+    p->swept_volume = 0;
+    p->lipschitz_corrected = 0;
+    p->fastenabled = 1;
+    p->fast_multiplier = 1.0f;
+    p->swept_window = 1.0f / 60.0f;
+    p->voxel_capacity = 0;
+    p->queue_capacity = 0;
+}
+
+const char *tsi_strerror(int code) {
+    switch (code) {
+    case TSI_OK: return "ok";
+    case TSI_E_OOM: return "out of memory";
+    case TSI_E_QUEUE_FULL: return "frontier list overflow";
+    case TSI_E_POOL_EXHAUSTED: return "voxel pool exhausted";
+    case TSI_E_TAPE_TOO_LONG: return "tape longer than 4096 instructions";
+    case TSI_E_HIP: return "HIP runtime error";
+    case TSI_E_ARG: return "invalid argument";
+    case TSI_E_PARSE: return "cannot parse .ll text";
+    case TSI_E_NO_TAPE: return "no tape uploaded";
+    case TSI_E_UNSUPPORTED: return "tape needs more live values than the register file holds";
+    case TSI_E_BUDGET: return "request exceeds the admission budget";
+    default: return "unknown error";
+    }
+}
+
+float tsi_camera_zdist(uint32_t height) {
+    // compute.wgsl:4592: z = f32(res.y) / tan(radians(vfov) / 2.0), radians per 2432, fov 90.
+    const float rad = 90.0f * 0.017453292519943295474f;
+    return float(height) / std::tan(rad / 2.0f);
+}
+
+int tsi_create(const tsi_params *params, tsi_ctx **out) {
+    if (!params || !out) return TSI_E_ARG;
+    *out = nullptr;
+    if (params->localsubdiv < 1 || params->localsubdiv > kMaxLevels - 4) return TSI_E_ARG;
+    tsi_ctx *c = new (std::nothrow) tsi_ctx();
+    if (!c) return TSI_E_OOM;
+    c->p = *params;
+    int rc = TSI_OK;
+    do {
+        if (hipGetDevice(&c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
+        c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = TSI_E_HIP; break; }
+        bool evok = true;
+        for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
+        if (!evok) { rc = TSI_E_HIP; break; }
+        uint64_t nodes = params->voxel_capacity ? params->voxel_capacity : (uint64_t(1) << 24);
+        nodes = (nodes + 7) / 8 * 8;
+        if (nodes < 64) nodes = 64;
+        if (nodes / 8 > 0x7FFFFFF0ull) { rc = TSI_E_ARG; break; }
+        c->nodes = nodes;
+        c->nblocks = uint32_t(nodes / 8);
+        uint64_t lc = params->queue_capacity ? params->queue_capacity : nodes / 4;
+        if (lc > 0xFFFFFFF0ull) lc = 0xFFFFFFF0ull;
+        c->list_cap = uint32_t(lc < 16 ? 16 : lc);
+        c->dlist_cap = c->nblocks;
+        if ((rc = dmalloc(&c->st, nodes)) || (rc = dmalloc(&c->blo, nodes)) || (rc = dmalloc(&c->bhi, nodes)) ||
+            (rc = dmalloc(&c->grad, nodes)))
+            break;
+        for (auto &l : c->lists)
+            for (auto &x : l)
+                if (!rc) rc = dmalloc(&x, c->list_cap);
+        if (rc) break;
+        if ((rc = dmalloc(&c->dlists[0], c->dlist_cap)) || (rc = dmalloc(&c->dlists[1], c->dlist_cap)) ||
+            (rc = dmalloc(&c->pending, c->nblocks)) || (rc = dmalloc(&c->free_stack, c->nblocks)) ||
+            (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->visits, 1)))
+            break;
+        if (hipMemsetAsync(c->cnt, 0, sizeof(Counters), c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
+        c->grid_fwd = uint32_t(c->num_cu) * 8u;
+        c->grid_rev = uint32_t(c->num_cu) * 2u;
+        c->grid_misc = uint32_t(c->num_cu) * 4u;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
+    } while (0);
+    if (rc != TSI_OK) {
+        release(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return TSI_OK;
+}
+
+void tsi_destroy(tsi_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+}
+
+int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
+    if (!c || !tape) return TSI_E_ARG;
+    if (n > 4096) return TSI_E_TAPE_TOO_LONG;
+    std::vector<FwdIns> fwd;
+    std::vector<RevIns> rev;
+    Lowered L;
+    int rc;
+    try {
+        rc = lower(tape, n, fwd, rev, L);
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    if (rc != TSI_OK) return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    dfree(c->fprog);
+    dfree(c->rprog);
+    if ((rc = dmalloc(&c->fprog, fwd.size())) || (rc = dmalloc(&c->rprog, rev.size()))) return rc;
+    HIP_OK(hipMemcpy(c->fprog, fwd.data(), fwd.size() * sizeof(FwdIns), hipMemcpyHostToDevice));
+    if (!rev.empty()) HIP_OK(hipMemcpy(c->rprog, rev.data(), rev.size() * sizeof(RevIns), hipMemcpyHostToDevice));
+    size_t need = size_t(c->grid_rev) * 4u * 64u * size_t(L.stack ? L.stack : 1) * sizeof(float2);
+    if (need > c->stack_bytes) {
+        dfree(c->stack);
+        c->stack_bytes = 0;
+        if ((rc = dmalloc(&c->stack, need / sizeof(float2)))) return rc;
+        c->stack_bytes = need;
+    }
+    c->tape.assign(tape, tape + n);
+    c->low = L;
+    c->have_tape = true;
+    return TSI_OK;
+}
+
+int tsi_reset(tsi_ctx *c, const float root_box[6]) {
+    if (!c || !root_box) return TSI_E_ARG;
+    KArgs a = make_args(c);
+    if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TSI_OK;
+}
+
+int tsi_set_uniforms(tsi_ctx *c, const tsi_uniforms *u) {
+    if (!c || !u) return TSI_E_ARG;
+    c->u = *u;
+    return TSI_OK;
+}
+
+int tsi_enqueue_root(tsi_ctx *c) {
+    if (!c) return TSI_E_ARG;
+    if (launch_enqueue_root(make_args(c), c->stream) != 0) return TSI_E_HIP;
+    return TSI_OK;
+}
+
+int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
+    if (!c) return TSI_E_ARG;
+    if (!c->have_tape) return TSI_E_NO_TAPE;
+    KArgs a = make_args(c);
+    FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc};
+    HIP_OK(hipEventRecord(c->ev[0], c->stream));
+    if (launch_frame(a, fl, c->stream, c->ev[2], c->ev[3]) != 0) return TSI_E_HIP;
+    HIP_OK(hipEventRecord(c->ev[1], c->stream));
+    HIP_OK(hipEventSynchronize(c->ev[1]));
+    Counters h;
+    HIP_OK(hipMemcpy(&h, c->cnt, sizeof h, hipMemcpyDeviceToHost));
+    float ms = 0.0f, ems = 0.0f;
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&ems, c->ev[2], c->ev[3]);
+    if (gpu_ms) *gpu_ms = ms;
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->gpu_ms = ms;
+        stats->eval_ms = ems;
+        stats->cell_evals = h.stats[kStatCells];
+        stats->rev_evals = h.stats[kStatRevCells];
+        stats->fwd_ops = h.stats[kStatCells] * c->low.fwd_cost;
+        stats->rev_ops = h.stats[kStatRevOps];
+        stats->parents = h.stats[kStatParents];
+        stats->deallocs = h.stats[kStatDeallocs];
+        stats->enqueued = h.stats[kStatEnqueued];
+        stats->allocated_nodes = h.stats[kStatLive] * 8u;
+        stats->eval_launches = uint64_t(2 * c->p.localsubdiv);
+    }
+    if (h.frame_error & kErrPool) return TSI_E_POOL_EXHAUSTED;
+    if (h.frame_error & kErrQueue) return TSI_E_QUEUE_FULL;
+    return TSI_OK;
+}
+
+int tsi_trace(tsi_ctx *c, float *rgba_out, float *depth_out) {
+    if (!c) return TSI_E_ARG;
+    const uint32_t W = c->u.resolution[0], H = c->u.resolution[1];
+    if (W == 0 || H == 0) return TSI_E_ARG;
+    const size_t px = size_t(W) * H;
+    int rc;
+    if (px > c->frame_px) {
+        dfree(c->rgba);
+        dfree(c->depth);
+        c->frame_px = 0;
+        if ((rc = dmalloc(&c->rgba, px)) || (rc = dmalloc(&c->depth, px))) return rc;
+        c->frame_px = px;
+    }
+    TraceArgs t{};
+    std::memcpy(t.view, c->u.view, sizeof t.view);
+    t.width = W;
+    t.height = H;
+    t.time = c->u.time;
+    t.zdist = tsi_camera_zdist(H);
+    t.rgba = c->rgba;
+    t.depth = c->depth;
+    t.visits = c->visits;
+    HIP_OK(hipMemsetAsync(c->visits, 0, sizeof(unsigned long long), c->stream));
+    if (launch_trace(make_args(c), t, c->stream) != 0) return TSI_E_HIP;
+    if (rgba_out) HIP_OK(hipMemcpyAsync(rgba_out, c->rgba, px * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    if (depth_out) HIP_OK(hipMemcpyAsync(depth_out, c->depth, px * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long v = 0;
+    HIP_OK(hipMemcpyAsync(&v, c->visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->last_visits = v;
+    return TSI_OK;
+}
+
+int tsi_frame_device(tsi_ctx *c, const float **rgba_dev, const float **depth_dev) {
+    if (!c) return TSI_E_ARG;
+    if (rgba_dev) *rgba_dev = reinterpret_cast<const float *>(c->rgba);
+    if (depth_dev) *depth_dev = c->depth;
+    return TSI_OK;
+}
+
+int tsi_read_tape(tsi_ctx *c, tsi_gpu_instruction *out, uint32_t cap, uint32_t *n) {
+    if (!c) return TSI_E_ARG;
+    if (!c->have_tape) return TSI_E_NO_TAPE;
+    if (n) *n = uint32_t(c->tape.size());
+    if (out) {
+        uint32_t m = uint32_t(c->tape.size()) < cap ? uint32_t(c->tape.size()) : cap;
+        std::memcpy(out, c->tape.data(), m * sizeof(tsi_gpu_instruction));
+    }
+    return TSI_OK;
+}
+
+int tsi_read_nodes(tsi_ctx *c, tsi_node *out, uint64_t cap, uint64_t *count) {
+    if (!c) return TSI_E_ARG;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    Counters h;
+    HIP_OK(hipMemcpy(&h, c->cnt, sizeof h, hipMemcpyDeviceToHost));
+    const uint64_t used = (uint64_t(h.bump) + 1u) * 8u;
+    std::vector<float4> st, gr;
+    try {
+        st.resize(used);
+        gr.resize(used);
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    HIP_OK(hipMemcpy(st.data(), c->st, used * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(gr.data(), c->grad, used * sizeof(float4), hipMemcpyDeviceToHost));
+    struct Fr { uint32_t node, d, i, j, k; };
+    std::vector<Fr> stack{{0, 0, 0, 0, 0}};
+    uint64_t n = 0;
+    while (!stack.empty()) {
+        Fr f = stack.back();
+        stack.pop_back();
+        if (f.node >= used) return TSI_E_ARG;
+        const float4 s = st[f.node];
+        uint32_t meta;
+        std::memcpy(&meta, &s.w, 4);
+        const uint32_t child = meta & 0x7FFFFFFFu;
+        if (out && n < cap) {
+            tsi_node &o = out[n];
+            o.depth = f.d; o.i = f.i; o.j = f.j; o.k = f.k;
+            o.f = s.x; o.t = s.y; o.time_err = s.z;
+            o.fast = (meta & 0x80000000u) ? 1.0f : 0.0f;
+            o.has_children = child != 0;
+            o.dx = gr[f.node].x; o.dy = gr[f.node].y; o.dz = gr[f.node].z;
+        }
+        ++n;
+        if (child) {
+            for (uint32_t l = 0; l < 8; ++l)
+                stack.push_back({(child - 1u) * 8u + l, f.d + 1, 2 * f.i + (l & 1u), 2 * f.j + ((l >> 1) & 1u),
+                                 2 * f.k + (l >> 2)});
+        }
+    }
+    if (count) *count = n;
+    return TSI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// tsi_tape.h -- opcode numbering and the lowered (device) tape format, shared by the host
+// lowering pass and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+namespace tsi {
+
+// src/ir.rs:8-43 opcode numbering (kept: it is part of the 32-byte tape format).
+namespace Op {
+enum : uint32_t {
+    NOP = 0, ARGX = 1, ARGY = 2, ARGZ = 3, ARGW = 4, ADD = 5, SUB = 6, MUL = 7, DIV = 8, EXP = 9, LOG = 10,
+    SQR = 11, SQRT = 12, POW = 13, SIN = 14, COS = 15, NEG = 16, MIN = 17, MAX = 18, ABS = 19, CUP = 20,
+    CAP = 21, BR = 22, PHI = 23, TRUNC = 24, FLOOR = 25, CEIL = 26, SIGN = 27, RET = 28, MOD = 29, ATAN2 = 30
+};
+}
+
+// Dense "form" = (opcode, operand side) pairs the forward switch of compute.wgsl:3642-3764
+// distinguishes.  RR = both registers, RL = register/literal, LR = literal/register.
+enum Form : uint8_t {
+    F_ADD_RR = 0, F_SUB_RR, F_MUL_RR, F_DIV_RR, F_MIN_RR, F_MAX_RR, F_POW_RR, F_MOD_RR, F_ATAN2_RR, F_NOP_RR,
+    F_ADD_RL, F_SUB_RL, F_MUL_RL, F_DIV_RL, F_MIN_RL, F_MAX_RL, F_POW_RL, F_MOD_RL, F_ATAN2_RL,
+    F_SQR, F_SQRT, F_EXP, F_LOG, F_NEG, F_ABS, F_FLOOR, F_SIN, F_COS, F_RET,
+    F_SUB_LR, F_DIV_LR,
+    F_ZERO,        // lit/lit: value [0,0] (compute.wgsl:3765 `default: {}`)
+    F_NINF_RR,     // an opcode the forward switch does not know: I_FROM(-inf), both operands registers
+    F_NINF_RL,
+    F_NINF_LR,
+    F_COUNT
+};
+
+// Register-file size of the interpreter (values, then sensitivities): slots per lane.
+constexpr int kSlots = 32;
+constexpr uint8_t kNoSlot = 0xFF;
+
+// Forward instruction, 16 bytes.  Operands a/b and destination d are register-file slots.
+// push: stack position (in intervals) where the operand values the reverse rule needs are
+// saved; npush = how many (0, 1 = left, 2 = left and right, 3 = right only).
+struct FwdIns {
+    uint8_t form, a, b, d;
+    uint16_t push;
+    uint8_t npush;
+    uint8_t pad;
+    float lit;
+    uint32_t index;   // original instruction index (for pruning bitmasks)
+};
+
+// Reverse instruction, 16 bytes (executed for j = n-1 down to 4).
+enum : uint8_t {
+    R_SJ_ONE = 1,     // sens(j) = [1,1]: the output (compute.wgsl:3777)
+    R_SJ_ZERO = 2,    // sens(j) never accumulated: [0,0]
+    R_FT_A = 4,       // first touch of the left operand's sens: previous value is [+0,+0]
+    R_FT_B = 8,
+    R_U1_A = 16,      // uses(left operand) == 1 (short-circuit test, compute.wgsl:3802)
+    R_U1_B = 32
+};
+struct RevIns {
+    uint8_t form, sj, sa, sb;
+    uint8_t flags;
+    uint8_t side;
+    uint16_t pop;     // stack position of the saved operands (same as FwdIns::push)
+    float lit;
+    uint16_t cost;    // algorithmic f32 ops when not short-circuited
+    uint16_t index;   // original instruction index j
+};
+
+// Host-side result of lowering a tape.
+struct Lowered {
+    uint32_t n = 0;                 // instructions in the tape
+    uint32_t nfwd = 0, nrev = 0;    // = n - 4 each
+    uint32_t stack = 0;             // pushed intervals per lane
+    uint32_t slots = 0;             // max live values / sens
+    uint32_t arg_slot[4] = {0, 1, 2, 3};
+    uint32_t out_slot = 0;          // slot of the field value (instruction n-1)
+    uint64_t fwd_cost = 0;          // algorithmic f32 ops of one forward pass
+    int prunable = 0;               // every opcode is inclusion-monotone
+};
+
+}  // namespace tsi

@@ -1,0 +1,63 @@
+"""GPU parity: libtsi's set inversion and trace against the oracle, bit for bit.
+
+Every frame of a multi-frame temporal sequence is compared node by node (canonical
+(depth, i, j, k) keys; f, t, time_err, fast, has_children, dx, dy, dz bitwise).
+"""
+import numpy as np
+import pytest
+
+import temporalsetinversion_amd as T
+from helpers import assert_same_nodes, run_pair, tape_of
+
+pytestmark = pytest.mark.gpu
+
+SEQ = [k / 60.0 for k in range(0, 8)] + [0.5, 0.75, 1.0]
+
+
+@pytest.mark.parametrize("scene,L", [("sphere", 5), ("spheres_linear", 6), ("spheres", 6)])
+def test_frames_match_oracle(scene, L):
+    tape = tape_of(scene)
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ, localsubdiv=L):
+        print(f"scene={scene} L={L} frame={k} nodes={len(o)} cells={os_.cell_evals}", flush=True)
+        assert_same_nodes(g, o, f"{scene} frame {k}")
+
+
+@pytest.mark.parametrize("opts", [dict(swept_volume=True), dict(lipschitz_corrected=True),
+                                  dict(point_sample=False), dict(fastenabled=False),
+                                  dict(delta=0.05, fast_multiplier=0.5)])
+def test_param_variants(opts):
+    tape = tape_of("spheres")
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:6], localsubdiv=5, **opts):
+        print(f"opts={opts} frame={k} nodes={len(o)}", flush=True)
+        assert_same_nodes(g, o, f"{opts} frame {k}")
+
+
+def test_stats_match_oracle():
+    tape = tape_of("spheres")
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6):
+        s = gs
+        assert s["cell_evals"] == os_.cell_evals
+        assert s["rev_evals"] == os_.rev_evals
+        assert s["fwd_ops"] == os_.fwd_ops
+        assert s["rev_ops"] == os_.rev_ops
+        assert s["enqueued"] == os_.enqueued
+
+
+@pytest.mark.parametrize("scene,L", [("capsules", 4), ("csg", 4)])
+def test_big_tapes_match_oracle(scene, L):
+    tape = tape_of(scene)
+    for k, g, o, gs, os_, _, _ in run_pair(tape, [0.0, 1 / 60, 2 / 60, 0.3], localsubdiv=L):
+        print(f"scene={scene} frame={k} nodes={len(o)}", flush=True)
+        assert_same_nodes(g, o, f"{scene} frame {k}")
+
+
+def test_trace_matches_oracle():
+    tape = tape_of("spheres")
+    view = T.look_at_inverse((2.2, 1.7, 2.6))
+    for k, g, o, gs, os_, gf, of in run_pair(tape, [0.0, 1 / 60, 0.25], width=64, height=48, trace_every=1,
+                                                localsubdiv=6, view=view):
+        grgba, gdepth = gf
+        orgba, odepth = of
+        assert np.array_equal(grgba.view(np.uint32), orgba.view(np.uint32)), f"colour differs in frame {k}"
+        assert np.array_equal(gdepth.view(np.uint32), odepth.view(np.uint32)), f"depth differs in frame {k}"
+        assert (odepth > 0).sum() > 0
