@@ -24,6 +24,9 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # the tape interpreter's opcode dispatch is wave-uniform: keep it as scalar branches
+    # instead of exec-mask flow blocks
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
 ]
 
 

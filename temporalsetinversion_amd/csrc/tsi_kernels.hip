@@ -72,195 +72,284 @@ constexpr uint32_t kChildMask = 0x7FFFFFFFu;   // child block + 1, 0 = no childr
 // wave-uniform slot numbers so the compiler keeps it in VGPRs (s_set_gpr_idx moves).
 typedef float Slots[kSlots];
 
-// Forward pass over the lowered tape (compute.wgsl:3633-3772).  When PUSH, the operand
-// values the reverse rules need are saved to this wave's stack region.
+// Programs are read through the constant address space: the kernel never writes them, so the
+// loads become scalar (s_load) and the slot numbers land in SGPRs (no per-lane waterfall).
+typedef const __attribute__((address_space(4))) uint32_t *CWords;
+// Transcendental and rarely used forms live behind a call: their per-lane branches stay out of
+// the interpreter loop, whose dispatch then compiles to uniform scalar branches only.
+__device__ __noinline__ Iv rare_forward(uint32_t form, Iv l, Iv r, float c) {
+    switch (form) {
+    case F_SIN: return SIN_I(l);
+    case F_COS: return COS_I(l);
+    case F_EXP: return EXP_I(l);
+    case F_LOG: return LOG_I(l);
+    case F_FLOOR: return FLOOR_I(l);
+    case F_POW_RR: return POW_II(l, r);
+    case F_POW_RL: return POW_II(l, I_FROM(c));
+    case F_MOD_RR: return MOD_II(l, r);
+    case F_MOD_RL: return MOD_II(l, I_FROM(c));
+    case F_ATAN2_RR: return ATAN2_II(l, r);
+    default: return ATAN2_II(l, I_FROM(c));
+    }
+}
+
+// Reverse rules of the same forms (compute.wgsl:2806-2970); returns the rule for the left
+// operand in x0 and, for two-register forms, the right one in x1.
+__device__ __noinline__ void rare_reverse(uint32_t form, Iv l, Iv r, float c, Iv *x0, Iv *x1) {
+    const Iv ONE = I_FROM(1.0f);
+    switch (form) {
+    case F_SIN: *x0 = COS_I(l); break;
+    case F_COS: *x0 = NEG_I(SIN_I(l)); break;
+    case F_EXP: *x0 = EXP_I(l); break;
+    case F_LOG: *x0 = DIV_II(ONE, l); break;
+    case F_FLOOR: *x0 = FLOOR_I_RR(l); break;
+    case F_POW_RL: *x0 = MUL_II(I_FROM(c), POW_II(l, SUB_IF(I_FROM(c), 1.0f))); break;
+    case F_ATAN2_RL: *x0 = DIV_II(NEG_I(l), ADD_II(SQR_I(I_FROM(c)), SQR_I(l))); break;
+    case F_POW_RR:
+        *x0 = MUL_II(r, POW_II(l, SUB_IF(r, 1.0f)));
+        *x1 = MUL_II(POW_II(l, r), LOG_I(l));
+        break;
+    default: {   // F_ATAN2_RR
+        const Iv den = ADD_II(SQR_I(r), SQR_I(l));
+        *x0 = DIV_II(NEG_I(l), den);
+        *x1 = DIV_II(r, den);
+        break;
+    }
+    }
+}
+
+// One forward step (compute.wgsl:3642-3772): record words w0 = {form, a, b, d},
+// w1 = {push, npush}, w2 = literal.
 template <bool PUSH>
-TSI_DI Iv run_forward(const FwdIns *__restrict__ prog, uint32_t nfwd, Slots &Rlo, Slots &Rhi,
-                      float2 *__restrict__ stk) {
-    Iv v = I_FROM(0.0f);
-    for (uint32_t k = 0; k < nfwd; ++k) {
-        const FwdIns f = prog[k];
-        const Iv l = I(Rlo[f.a], Rhi[f.a]);
-        const Iv r = I(Rlo[f.b], Rhi[f.b]);
-        const float c = f.lit;
-        switch (f.form) {
-        case F_ADD_RR: v = ADD_II(l, r); break;
-        case F_ADD_RL: v = ADD_IF(l, c); break;
-        case F_SUB_RR: v = SUB_II(l, r); break;
-        case F_SUB_RL: v = SUB_IF(l, c); break;
-        case F_SUB_LR: v = SUB_II(I_FROM(c), r); break;
-        case F_MUL_RR: v = MUL_II(l, r); break;
-        case F_MUL_RL: v = MUL_IF(l, c); break;
-        case F_SQR: v = SQR_I(l); break;
-        case F_SQRT: v = SQRT_I(l); break;
-        case F_MIN_RR: v = MIN_II(l, r); break;
-        case F_MIN_RL: v = MIN_IF(l, c); break;
-        case F_MAX_RR: v = MAX_II(l, r); break;
-        case F_MAX_RL: v = MAX_IF(l, c); break;
-        case F_NEG: v = NEG_I(l); break;
-        case F_ABS: v = ABS_I(l); break;
-        case F_RET: v = l; break;
-        case F_DIV_RR: v = DIV_II(l, r); break;
-        case F_DIV_RL: v = DIV_IF(l, c); break;
-        case F_DIV_LR: v = DIV_II(I_FROM(c), r); break;
-        case F_SIN: v = SIN_I(l); break;
-        case F_COS: v = COS_I(l); break;
-        case F_EXP: v = EXP_I(l); break;
-        case F_LOG: v = LOG_I(l); break;
-        case F_FLOOR: v = FLOOR_I(l); break;
-        case F_POW_RR: v = POW_II(l, r); break;
-        case F_POW_RL: v = POW_II(l, I_FROM(c)); break;
-        case F_MOD_RR: v = MOD_II(l, r); break;
-        case F_MOD_RL: v = MOD_II(l, I_FROM(c)); break;
-        case F_ATAN2_RR: v = ATAN2_II(l, r); break;
-        case F_ATAN2_RL: v = ATAN2_II(l, I_FROM(c)); break;
-        case F_NOP_RR: v = I_FROM(c); break;
-        case F_ZERO: v = I_FROM(0.0f); break;
-        default: v = I_FROM(__uint_as_float(0xff800000u)); break;
-        }
-        v = fix(v);
-        Rlo[f.d] = v.lo;
-        Rhi[f.d] = v.hi;
-        if (PUSH && f.npush) {
-            float2 *p = stk + size_t(f.push) * 64;
-            if (f.npush == 3) {
+TSI_DI void fwd_step(uint32_t w0, uint32_t w1, uint32_t w2, Slots &Rlo, Slots &Rhi, float2 *__restrict__ stk) {
+    const uint32_t form = w0 & 0xFFu, ia = (w0 >> 8) & 0xFFu, ib = (w0 >> 16) & 0xFFu, id = w0 >> 24;
+    const Iv l = I(Rlo[ia], Rhi[ia]);
+    const Iv r = I(Rlo[ib], Rhi[ib]);
+    const float c = __uint_as_float(w2);
+    Iv v;
+    switch (form) {
+    case F_ADD_RR: v = ADD_II(l, r); break;
+    case F_ADD_RL: v = ADD_IF(l, c); break;
+    case F_SUB_RR: v = SUB_II(l, r); break;
+    case F_SUB_RL: v = SUB_IF(l, c); break;
+    case F_SUB_LR: v = SUB_II(I_FROM(c), r); break;
+    case F_MUL_RR: v = MUL_II(l, r); break;
+    case F_MUL_RL: v = MUL_IF(l, c); break;
+    case F_SQR: v = SQR_I(l); break;
+    case F_SQRT: v = SQRT_I(l); break;
+    case F_MIN_RR: v = MIN_II(l, r); break;
+    case F_MIN_RL: v = MIN_IF(l, c); break;
+    case F_MAX_RR: v = MAX_II(l, r); break;
+    case F_MAX_RL: v = MAX_IF(l, c); break;
+    case F_NEG: v = NEG_I(l); break;
+    case F_ABS: v = ABS_I(l); break;
+    case F_RET: v = l; break;
+    case F_DIV_RR: v = DIV_II(l, r); break;
+    case F_DIV_RL: v = DIV_IF(l, c); break;
+    case F_DIV_LR: v = DIV_II(I_FROM(c), r); break;
+    case F_SIN: case F_COS: case F_EXP: case F_LOG: case F_FLOOR: case F_POW_RR: case F_POW_RL:
+    case F_MOD_RR: case F_MOD_RL: case F_ATAN2_RR: case F_ATAN2_RL:
+        v = rare_forward(form, l, r, c);
+        break;
+    case F_NOP_RR: v = I_FROM(c); break;
+    case F_ZERO: v = I_FROM(0.0f); break;
+    default: v = I_FROM(__uint_as_float(0xff800000u)); break;
+    }
+    v = fix(v);
+    Rlo[id] = v.lo;
+    Rhi[id] = v.hi;
+    if (PUSH) {
+        const uint32_t npush = (w1 >> 16) & 0xFFu;
+        if (npush) {
+            float2 *p = stk + size_t(w1 & 0xFFFFu) * 64;
+            if (npush == 3) {
                 p[0] = make_float2(r.lo, r.hi);
             } else {
                 p[0] = make_float2(l.lo, l.hi);
-                if (f.npush == 2) p[64] = make_float2(r.lo, r.hi);
+                if (npush == 2) p[64] = make_float2(r.lo, r.hi);
             }
+        }
+    }
+}
+
+// Programs are fetched in 64-byte batches of four 16-byte records, one batch ahead of the
+// batch being executed, so the scalar-load latency overlaps four interpreted instructions.
+// Programs are padded with zero records to a multiple of four plus one batch.
+constexpr uint32_t kBatch = 4;
+
+// Forward pass over the lowered tape (compute.wgsl:3633-3772).  When PUSH, the operand
+// values the reverse rules need are saved to this wave's stack region.
+template <bool PUSH>
+TSI_DI void run_forward(const FwdIns *prog, uint32_t nfwd, Slots &Rlo, Slots &Rhi, float2 *__restrict__ stk) {
+    CWords base = (CWords)prog;
+    uint32_t cur[4 * kBatch];
+#pragma unroll
+    for (int i = 0; i < int(4 * kBatch); ++i) cur[i] = base[i];
+    for (uint32_t k = 0; k < nfwd; k += kBatch) {
+        uint32_t nxt[4 * kBatch];
+        CWords nb = base + size_t(k + kBatch) * 4u;
+#pragma unroll
+        for (int i = 0; i < int(4 * kBatch); ++i) nxt[i] = nb[i];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i)
+            if (k + i < nfwd) fwd_step<PUSH>(cur[4 * i], cur[4 * i + 1], cur[4 * i + 2], Rlo, Rhi, stk);
+#pragma unroll
+        for (int i = 0; i < int(4 * kBatch); ++i) cur[i] = nxt[i];
+    }
+}
+
+// One reverse step (compute.wgsl:3786-3963): w0 = {form, sj, sa, sb}, w1 = {flags, side, pop},
+// w2 = literal, w3 = {cost, npush}; ops_in = the saved operand values (left.lo, left.hi, right.lo,
+// right.hi; a single saved interval sits in .xy).  Returns the op count when not short-circuited.
+TSI_DI uint32_t rev_step(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, Slots &Slo, Slots &Shi,
+                         float4 ops_in) {
+    const Iv Z = I_FROM(0.0f), ONE = I_FROM(1.0f), NINF = I_FROM(__uint_as_float(0xff800000u));
+    const uint32_t form = w0 & 0xFFu, isj = (w0 >> 8) & 0xFFu, isa = (w0 >> 16) & 0xFFu, isb = w0 >> 24;
+    const uint32_t flags = w1 & 0xFFu, side = (w1 >> 8) & 0xFFu, pop = w1 >> 16;
+    const float c = __uint_as_float(w2);
+    const uint32_t cost = w3 & 0xFFFFu;
+    Iv sj = (flags & R_SJ_ONE) ? ONE : Z;
+    if (!(flags & (R_SJ_ONE | R_SJ_ZERO))) sj = I(Slo[isj], Shi[isj]);
+    const bool sj0 = sj.lo == 0.0f && sj.hi == 0.0f;
+    uint32_t ops = 0;
+    (void)pop;
+    if (side == 1) {
+        const Iv l = I(ops_in.x, ops_in.y);
+        Iv x;
+        switch (form) {
+        case F_SQR: x = MUL_IF(l, 2.0f); break;
+        case F_SQRT: {
+            x = DIV_II(ONE, MUL_IF(SQRT_I(l), 2.0f));
+            x.hi = wmin(x.hi, kBig);
+            break;
+        }
+        case F_NEG: x = I_FROM(-1.0f); break;
+        case F_SIN: case F_COS: case F_EXP: case F_LOG: case F_FLOOR: case F_POW_RL: case F_ATAN2_RL: {
+            Iv unused;
+            rare_reverse(form, l, Z, c, &x, &unused);
+            break;
+        }
+        case F_ADD_RL: case F_SUB_RL: case F_RET: case F_MOD_RL: x = ONE; break;
+        case F_MUL_RL: x = I_FROM(c); break;
+        case F_DIV_RL: x = DIV_II(ONE, I_FROM(c)); break;
+        case F_ABS: x = ABS_I_RR(l); break;
+        case F_MIN_RL: x = sel(l.hi < c, ONE, sel(c < l.lo, Z, ONE)); break;
+        case F_MAX_RL: x = sel(l.hi < c, Z, ONE); break;
+        default: x = NINF; break;
+        }
+        const Iv old = (flags & R_FT_A) ? Z : I(Slo[isa], Shi[isa]);
+        Iv nv = ADD_II(MUL_II(fix(x), sj), old);
+        const bool sc = sj0 && (flags & R_U1_A);
+        nv = sel(sc, Z, nv);
+        ops = sc ? 0u : cost;
+        Slo[isa] = nv.lo;
+        Shi[isa] = nv.hi;
+    } else if (side == 0) {
+        const Iv l = I(ops_in.x, ops_in.y), r = I(ops_in.z, ops_in.w);
+        Iv x0, x1;
+        switch (form) {
+        case F_ADD_RR: x0 = ONE; x1 = ONE; break;
+        case F_SUB_RR: x0 = ONE; x1 = I_FROM(-1.0f); break;
+        case F_MUL_RR: x0 = r; x1 = l; break;
+        case F_MIN_RR: {   // MIN_II_RR (compute.wgsl:2884)
+            const bool lt = l.hi < r.lo, gt = r.hi < l.lo;
+            x0 = sel(lt, ONE, sel(gt, Z, ONE));
+            x1 = sel(lt, Z, ONE);
+            break;
+        }
+        case F_MAX_RR: {   // MAX_II_RR (compute.wgsl:2894)
+            const bool lt = l.hi < r.lo, gt = r.hi < l.lo;
+            x0 = sel(lt, Z, ONE);
+            x1 = sel(lt, ONE, sel(gt, Z, ONE));
+            break;
+        }
+        case F_DIV_RR: x0 = DIV_II(ONE, r); x1 = NEG_I(DIV_II(l, SQR_I(r))); break;
+        case F_POW_RR: case F_ATAN2_RR: rare_reverse(form, l, r, c, &x0, &x1); break;
+        case F_NOP_RR: x0 = Z; x1 = Z; break;
+        default: x0 = NINF; x1 = NINF; break;
+        }
+        const Iv olda = (flags & R_FT_A) ? Z : I(Slo[isa], Shi[isa]);
+        const Iv oldb = (flags & R_FT_B) ? Z : I(Slo[isb], Shi[isb]);
+        Iv na = ADD_II(MUL_II(fix(x0), sj), olda);
+        Iv nb = ADD_II(MUL_II(fix(x1), sj), oldb);
+        const bool sc = sj0 && (flags & R_U1_A) && (flags & R_U1_B);
+        na = sel(sc, Z, na);
+        nb = sel(sc, Z, nb);
+        ops = sc ? 0u : cost;
+        Slo[isa] = na.lo;
+        Shi[isa] = na.hi;
+        Slo[isb] = nb.lo;
+        Shi[isb] = nb.hi;
+    } else if (side == 2) {
+        const Iv r = I(ops_in.x, ops_in.y);
+        Iv x;
+        switch (form) {
+        case F_SUB_LR: x = I_FROM(-1.0f); break;
+        case F_DIV_LR: x = NEG_I(DIV_II(I_FROM(c), SQR_I(r))); break;
+        default: x = NINF; break;
+        }
+        const Iv old = (flags & R_FT_B) ? Z : I(Slo[isb], Shi[isb]);
+        Iv nv = ADD_II(MUL_II(fix(x), sj), old);
+        const bool sc = sj0 && (flags & R_U1_B);
+        nv = sel(sc, Z, nv);
+        ops = sc ? 0u : cost;
+        Slo[isb] = nv.lo;
+        Shi[isb] = nv.hi;
+    }
+    return ops;
+}
+
+// Saved operands of one reverse record: npush (w3 bits 16-23) intervals at position pop.
+TSI_DI float4 load_saved(uint32_t w1, uint32_t w3, const float2 *__restrict__ stk) {
+    const uint32_t npush = (w3 >> 16) & 0xFFu;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (npush) {
+        const float2 *p = stk + size_t(w1 >> 16) * 64;
+        const float2 q = p[0];
+        v.x = q.x;
+        v.y = q.y;
+        if (npush == 2) {
+            const float2 w = p[64];
+            v.z = w.x;
+            v.w = w.y;
         }
     }
     return v;
 }
 
 // Reverse pass (compute.wgsl:3780-3964) with sensitivities in the register file; returns the
-// algorithmic op count of the non-short-circuited steps of this lane.
-TSI_DI uint32_t run_reverse(const RevIns *__restrict__ prog, uint32_t nrev, Slots &Slo, Slots &Shi,
+// algorithmic op count of the non-short-circuited steps of this lane.  Two-deep pipeline: the
+// saved operands of batch b+1 are loaded from the stack while batch b executes, and the
+// program words of batch b+2 are fetched meanwhile.
+TSI_DI uint32_t run_reverse(const RevIns *prog, uint32_t nrev, Slots &Slo, Slots &Shi,
                             const float2 *__restrict__ stk) {
-    const Iv Z = I_FROM(0.0f), ONE = I_FROM(1.0f), NINF = I_FROM(__uint_as_float(0xff800000u));
     uint32_t ops = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { Slo[k] = 0.0f; Shi[k] = 0.0f; }
-    for (uint32_t k = 0; k < nrev; ++k) {
-        const RevIns ri = prog[k];
-        Iv sj;
-        if (ri.flags & R_SJ_ONE) sj = ONE;
-        else if (ri.flags & R_SJ_ZERO) sj = Z;
-        else sj = I(Slo[ri.sj], Shi[ri.sj]);
-        const bool sj0 = sj.lo == 0.0f && sj.hi == 0.0f;
-        const float c = ri.lit;
-        const float2 *p = stk + size_t(ri.pop) * 64;
-        if (ri.side == 1) {
-            Iv l = Z;
-            switch (ri.form) {
-            case F_SQR: case F_SQRT: case F_SIN: case F_COS: case F_EXP: case F_LOG: case F_FLOOR: case F_ABS:
-            case F_MIN_RL: case F_MAX_RL: case F_POW_RL: case F_ATAN2_RL: {
-                float2 q = p[0];
-                l = I(q.x, q.y);
-                break;
-            }
-            default: break;
-            }
-            Iv x;
-            switch (ri.form) {
-            case F_SQR: x = MUL_IF(l, 2.0f); break;
-            case F_SQRT: {
-                x = DIV_II(ONE, MUL_IF(SQRT_I(l), 2.0f));
-                x.hi = wmin(x.hi, kBig);
-                break;
-            }
-            case F_NEG: x = I_FROM(-1.0f); break;
-            case F_SIN: x = COS_I(l); break;
-            case F_COS: x = NEG_I(SIN_I(l)); break;
-            case F_ADD_RL: case F_SUB_RL: case F_RET: case F_MOD_RL: x = ONE; break;
-            case F_MUL_RL: x = I_FROM(c); break;
-            case F_DIV_RL: x = DIV_II(ONE, I_FROM(c)); break;
-            case F_EXP: x = EXP_I(l); break;
-            case F_LOG: x = DIV_II(ONE, l); break;
-            case F_FLOOR: x = FLOOR_I_RR(l); break;
-            case F_ABS: x = ABS_I_RR(l); break;
-            case F_MIN_RL: x = (l.hi < c) ? ONE : ((c < l.lo) ? Z : ONE); break;
-            case F_MAX_RL: x = (l.hi < c) ? Z : ((c < l.lo) ? ONE : ONE); break;
-            case F_POW_RL: x = MUL_II(I_FROM(c), POW_II(l, SUB_IF(I_FROM(c), 1.0f))); break;
-            case F_ATAN2_RL: x = DIV_II(NEG_I(l), ADD_II(SQR_I(I_FROM(c)), SQR_I(l))); break;
-            default: x = NINF; break;
-            }
-            const Iv old = (ri.flags & R_FT_A) ? Z : I(Slo[ri.sa], Shi[ri.sa]);
-            Iv nv = ADD_II(MUL_II(fix(x), sj), old);
-            const bool sc = sj0 && (ri.flags & R_U1_A);
-            if (sc) nv = Z;
-            ops += sc ? 0u : uint32_t(ri.cost);
-            Slo[ri.sa] = nv.lo;
-            Shi[ri.sa] = nv.hi;
-        } else if (ri.side == 0) {
-            Iv l = Z, r = Z;
-            switch (ri.form) {
-            case F_MIN_RR: case F_MAX_RR: case F_MUL_RR: case F_DIV_RR: case F_POW_RR: case F_ATAN2_RR: {
-                float2 q = p[0], w = p[64];
-                l = I(q.x, q.y);
-                r = I(w.x, w.y);
-                break;
-            }
-            default: break;
-            }
-            Iv x0, x1;
-            switch (ri.form) {
-            case F_ADD_RR: x0 = ONE; x1 = ONE; break;
-            case F_SUB_RR: x0 = ONE; x1 = I_FROM(-1.0f); break;
-            case F_MUL_RR: x0 = r; x1 = l; break;
-            case F_MIN_RR:
-                if (l.hi < r.lo) { x0 = ONE; x1 = Z; }
-                else if (r.hi < l.lo) { x0 = Z; x1 = ONE; }
-                else { x0 = ONE; x1 = ONE; }
-                break;
-            case F_MAX_RR:
-                if (l.hi < r.lo) { x0 = Z; x1 = ONE; }
-                else if (r.hi < l.lo) { x0 = ONE; x1 = Z; }
-                else { x0 = ONE; x1 = ONE; }
-                break;
-            case F_DIV_RR: x0 = DIV_II(ONE, r); x1 = NEG_I(DIV_II(l, SQR_I(r))); break;
-            case F_POW_RR: x0 = MUL_II(r, POW_II(l, SUB_IF(r, 1.0f))); x1 = MUL_II(POW_II(l, r), LOG_I(l)); break;
-            case F_ATAN2_RR: {
-                Iv den = ADD_II(SQR_I(r), SQR_I(l));
-                x0 = DIV_II(NEG_I(l), den);
-                x1 = DIV_II(r, den);
-                break;
-            }
-            case F_NOP_RR: x0 = Z; x1 = Z; break;
-            default: x0 = NINF; x1 = NINF; break;
-            }
-            const Iv olda = (ri.flags & R_FT_A) ? Z : I(Slo[ri.sa], Shi[ri.sa]);
-            const Iv oldb = (ri.flags & R_FT_B) ? Z : I(Slo[ri.sb], Shi[ri.sb]);
-            Iv na = ADD_II(MUL_II(fix(x0), sj), olda);
-            Iv nb = ADD_II(MUL_II(fix(x1), sj), oldb);
-            const bool sc = sj0 && (ri.flags & R_U1_A) && (ri.flags & R_U1_B);
-            if (sc) { na = Z; nb = Z; }
-            ops += sc ? 0u : uint32_t(ri.cost);
-            Slo[ri.sa] = na.lo;
-            Shi[ri.sa] = na.hi;
-            Slo[ri.sb] = nb.lo;
-            Shi[ri.sb] = nb.hi;
-        } else if (ri.side == 2) {
-            Iv r = Z;
-            if (ri.form == F_DIV_LR) {
-                float2 w = p[0];
-                r = I(w.x, w.y);
-            }
-            Iv x;
-            switch (ri.form) {
-            case F_SUB_LR: x = I_FROM(-1.0f); break;
-            case F_DIV_LR: x = NEG_I(DIV_II(I_FROM(c), SQR_I(r))); break;
-            default: x = NINF; break;
-            }
-            const Iv old = (ri.flags & R_FT_B) ? Z : I(Slo[ri.sb], Shi[ri.sb]);
-            Iv nv = ADD_II(MUL_II(fix(x), sj), old);
-            const bool sc = sj0 && (ri.flags & R_U1_B);
-            if (sc) nv = Z;
-            ops += sc ? 0u : uint32_t(ri.cost);
-            Slo[ri.sb] = nv.lo;
-            Shi[ri.sb] = nv.hi;
-        }
+    CWords base = (CWords)prog;
+    uint32_t cur[4 * kBatch], nxt[4 * kBatch];
+    float4 scur[kBatch];
+#pragma unroll
+    for (int i = 0; i < int(4 * kBatch); ++i) cur[i] = base[i];
+#pragma unroll
+    for (int i = 0; i < int(4 * kBatch); ++i) nxt[i] = base[4 * kBatch + i];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) scur[i] = load_saved(cur[4 * i + 1], cur[4 * i + 3], stk);
+    for (uint32_t k = 0; k < nrev; k += kBatch) {
+        float4 snxt[kBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) snxt[i] = load_saved(nxt[4 * i + 1], nxt[4 * i + 3], stk);
+        uint32_t nn[4 * kBatch];
+        CWords nb = base + size_t(k + 2 * kBatch) * 4u;
+#pragma unroll
+        for (int i = 0; i < int(4 * kBatch); ++i) nn[i] = nb[i];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i)
+            if (k + i < nrev) ops += rev_step(cur[4 * i], cur[4 * i + 1], cur[4 * i + 2], cur[4 * i + 3], Slo, Shi, scur[i]);
+#pragma unroll
+        for (int i = 0; i < int(4 * kBatch); ++i) { cur[i] = nxt[i]; nxt[i] = nn[i]; }
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) scur[i] = snxt[i];
     }
     return ops;
 }
@@ -307,7 +396,9 @@ __global__ void __launch_bounds__(256) k_eval(KArgs a, int level, float time) {
     const uint32_t lane = lane_id();
     const uint32_t ps = lane >> 3, o = lane & 7u;
     const uint32_t lx = o & 1u, ly = (o >> 1) & 1u, lz = o >> 2;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave index made visibly uniform: the grid-stride loop and everything inside it then
+    // compile as scalar control flow
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     float2 *stk = REV ? a.stack + size_t(wave) * a.stack_len * 64 + lane : nullptr;
     const float eps = a.epsilon;
@@ -391,7 +482,7 @@ __global__ void __launch_bounds__(256) k_eval(KArgs a, int level, float time) {
         Iv field;
         const uint64_t need_eval = ballot(guard);
         if (need_eval) {
-            field = run_forward<REV>(a.fprog, a.nfwd, Rlo, Rhi, stk);
+            run_forward<REV>(a.fprog, a.nfwd, Rlo, Rhi, stk);
             field = I(Rlo[a.out_slot], Rhi[a.out_slot]);
         } else {
             field = I_FROM(0.0f);
@@ -494,7 +585,9 @@ __global__ void __launch_bounds__(256) k_dealloc(KArgs a, int step) {
     uint32_t *out = a.dlists[(step + 1) & 1];
     const uint32_t lane = lane_id();
     const uint32_t o = lane & 7u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave index made visibly uniform: the grid-stride loop and everything inside it then
+    // compile as scalar control flow
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     uint64_t freed = 0;
     for (uint32_t g = wave; g * 8u < count; g += nwaves) {

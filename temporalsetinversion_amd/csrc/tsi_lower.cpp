@@ -227,7 +227,7 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
         r.side = uint8_t(side);
         r.pop = f.push;
         r.lit = g.literal;
-        r.index = uint16_t(j);
+        r.npush = f.npush;
         r.cost = uint16_t(rev_cost(op, side));
         r.sa = r.sb = 0;
         if (j == n - 1) {

@@ -232,9 +232,13 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     HIP_OK(hipStreamSynchronize(c->stream));
     dfree(c->fprog);
     dfree(c->rprog);
+    // zero padding records: the kernels fetch programs in batches of four records, one batch
+    // ahead of execution
+    fwd.resize((fwd.size() + 3) / 4 * 4 + 8, FwdIns{});
+    rev.resize((rev.size() + 3) / 4 * 4 + 8, RevIns{});
     if ((rc = dmalloc(&c->fprog, fwd.size())) || (rc = dmalloc(&c->rprog, rev.size()))) return rc;
     HIP_OK(hipMemcpy(c->fprog, fwd.data(), fwd.size() * sizeof(FwdIns), hipMemcpyHostToDevice));
-    if (!rev.empty()) HIP_OK(hipMemcpy(c->rprog, rev.data(), rev.size() * sizeof(RevIns), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->rprog, rev.data(), rev.size() * sizeof(RevIns), hipMemcpyHostToDevice));
     size_t need = size_t(c->grid_rev) * 4u * 64u * size_t(L.stack ? L.stack : 1) * sizeof(float2);
     if (need > c->stack_bytes) {
         dfree(c->stack);

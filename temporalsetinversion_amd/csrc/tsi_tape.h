@@ -60,8 +60,11 @@ struct RevIns {
     uint16_t pop;     // stack position of the saved operands (same as FwdIns::push)
     float lit;
     uint16_t cost;    // algorithmic f32 ops when not short-circuited
-    uint16_t index;   // original instruction index j
+    uint8_t npush;    // saved operand intervals at `pop` (FwdIns::npush of the same instruction)
+    uint8_t pad;
 };
+static_assert(sizeof(FwdIns) == 16, "16-byte forward records");
+static_assert(sizeof(RevIns) == 16, "16-byte reverse records");
 
 // Host-side result of lowering a tape.
 struct Lowered {
