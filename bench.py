@@ -32,13 +32,13 @@ PEAK_VALU_UNPACKED_TOPS = 39.3
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
-    # plain method on one MI355X
+    # plain method on one MI355X: 512^3, 1024x1024, 64 capsules
     "capsules64_L9_1024": dict(scene="capsules", seed=1, localsubdiv=9, width=1024, height=1024,
                                lipschitz_corrected=True, swept_volume=False, point_sample=True),
-    # plain method, large (1024^3, swept, point-sampled)
+    # plain method, large: 1024^3, 1920x1080, 256-primitive CSG, swept, point-sampled
     "csg256_L10_1920x1080": dict(scene="csg", seed=2, localsubdiv=10, width=1920, height=1080,
                                  lipschitz_corrected=False, swept_volume=True, point_sample=True),
-    # configs[0] at GPU scale: 8 moving spheres
+    # the 8-moving-spheres field at 64^3
     "spheres8_L6_256": dict(scene="spheres", seed=0, localsubdiv=6, width=256, height=256,
                             lipschitz_corrected=False, swept_volume=False, point_sample=True),
 }
@@ -78,8 +78,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by octree "
-              f"subtree, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs the "
+              f"set-inversion path on a single MI355X (it would shard by octree subtree)", file=sys.stderr)
         sys.exit(2)
 
     import temporalsetinversion_amd as T

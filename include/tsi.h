@@ -66,11 +66,18 @@ typedef struct {
     uint64_t voxel_capacity;      /* nodes in the pool (0 = default) */
     uint64_t queue_capacity;      /* entries per frontier list (0 = voxel_capacity / 8) */
     uint32_t flags;               /* TSI_FLAG_* */
-    uint32_t reserved;
+    uint32_t prune_capacity;      /* frontier entries per list and level that get a pruned
+                                     program (0 = min(queue_capacity, 2^19)) */
 } tsi_params;
 
 enum {
-    TSI_FLAG_NO_PRUNE = 1u        /* evaluate the full tape in every cell (the reference's behaviour) */
+    /* Evaluate the full tape in every cell (the reference's behaviour).  Without it, a cell
+     * runs the tape pruned on its parent's MIN/MAX choices (compute.wgsl:2884-2903): results
+     * are identical for tapes of +, -, *, /literal, sqrt, min, max, abs, neg, floor; for tapes
+     * that add sin, cos, exp or log they are identical unless an interval bound is within a
+     * few ulp of a MIN/MAX decision; tapes with any other opcode are never pruned. */
+    TSI_FLAG_NO_PRUNE = 1u,
+    TSI_FLAG_PRUNE_EXACT_ONLY = 2u  /* prune only tapes of the first (exact) kind */
 };
 
 /* Per-frame statistics.  fwd_ops/rev_ops count algorithmic f32 interval operations. */
@@ -79,7 +86,7 @@ typedef struct {
     double eval_ms;               /* of which: tape-evaluation kernels */
     uint64_t cell_evals;          /* child cells that passed the evaluation guard (compute.wgsl:3629) */
     uint64_t rev_evals;           /* of those, cells that ran the reverse pass (compute.wgsl:3780) */
-    uint64_t fwd_ops;
+    uint64_t fwd_ops;             /* of the records actually run (pruned programs run fewer) */
     uint64_t rev_ops;
     uint64_t parents;             /* non-dealloc work items (parents subdivided) */
     uint64_t deallocs;            /* node blocks freed by the dealloc cascade */
@@ -87,6 +94,7 @@ typedef struct {
     uint64_t allocated_nodes;     /* live pool nodes after the frame */
     uint64_t eval_launches;
     uint64_t pruned_instr;        /* sum over evaluated cells of instructions skipped by pruning */
+    uint64_t prune_launches;      /* group-program generator launches */
 } tsi_stats;
 
 /* Canonical node record (see tsi_read_nodes). */

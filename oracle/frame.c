@@ -1,6 +1,6 @@
 /*
- * frame.c -- rows C (classification/subdivision), E (time error), D (trace) and U of the
- * oracle.  TEST INFRASTRUCTURE ONLY (see tsi_oracle.h).
+ * frame.c -- classification/subdivision, time error, octree trace and frame protocol of
+ * the oracle.  TEST INFRASTRUCTURE ONLY (see tsi_oracle.h).
  *
  * orc_set_inversion restates one dispatch of compute.wgsl `main` (3513-4056) executed by a
  * single workgroup: the broker queue (3249-3282) becomes a FIFO, mallocV/freeV (3437-3460)
@@ -231,9 +231,12 @@ static void process_parent(orc_state *s, uint32_t P, float time, float epsilon, 
         iv df = I_FROM(0.0f), df_dt = I_FROM(0.0f);
         if (take_derivative) {
             iv *sens = s->sens;
-            cv->dx = MID(sens[0]);                                        /* 3967 */
-            cv->dy = MID(sens[1]);
-            cv->dz = MID(sens[2]);
+            /* 3967-3969.  "+ 0.0f" turns a -0 into +0: the sign of a zero sensitivity is not
+             * part of the result (it depends on the order in which zero contributions are
+             * summed, which evaluating a pruned tape changes). */
+            cv->dx = MID(sens[0]) + 0.0f;
+            cv->dy = MID(sens[1]) + 0.0f;
+            cv->dz = MID(sens[2]) + 0.0f;
             df_dt = sens[3];                                              /* 3972 */
             float dt = DIAMETER(s->vals[3]);
             df = MUL_IF(df_dt, dt);
@@ -298,7 +301,7 @@ int orc_set_inversion(orc_state *s, float time, orc_stats *st)
     return s->status;
 }
 
-/* ---------------- trace (row D) ---------------- */
+/* ---------------- trace (rt_main) ---------------- */
 
 typedef struct { float x, y, z; } v3;
 static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }

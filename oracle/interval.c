@@ -1,6 +1,6 @@
 /*
- * interval.c -- rows A (forward), B (reverse + choices), P (choice reachability) of the
- * oracle.  TEST INFRASTRUCTURE ONLY (see tsi_oracle.h).
+ * interval.c -- forward interval evaluation, reverse AD with MIN/MAX choices and choice
+ * reachability of the oracle.  TEST INFRASTRUCTURE ONLY (see tsi_oracle.h).
  *
  * Each function restates the compute.wgsl function of the same name (line cited).
  * Intervals are not outward rounded (the reference does not round outward either).

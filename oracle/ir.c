@@ -1,5 +1,5 @@
 /*
- * ir.c -- row T of the oracle: restatement of src/ir.rs (the .ll -> tape builder).
+ * ir.c -- tape builder of the oracle: restatement of src/ir.rs (the .ll -> tape builder).
  * TEST INFRASTRUCTURE ONLY (see tsi_oracle.h).
  *
  * Mirrors, in order:

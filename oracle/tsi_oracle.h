@@ -14,7 +14,7 @@
  *   src/main.rs          first-frame seeding 730-766, frame protocol 639-690, 724-787
  *
  * Arithmetic conventions (the reference leaves them to the Vulkan driver; the
- * oracle and the HIP kernels both follow these, see DESIGN.md "numerics"):
+ * oracle and the HIP kernels both follow these):
  *   - IEEE-754 binary32, round-to-nearest-even, no FMA contraction, denormals kept;
  *   - WGSL min/max = IEEE-754-2019 minimumNumber/maximumNumber (a NaN operand
  *     yields the other operand, -0 orders below +0);
@@ -80,7 +80,7 @@ typedef struct {
 
 enum { ORC_OK = 0, ORC_E_PARSE = -1, ORC_E_TAPE_TOO_LONG = -2, ORC_E_POOL = -3, ORC_E_QUEUE = -4, ORC_E_ARG = -5 };
 
-/* ---- row T: tape builder (ir.rs:190-366) ---- */
+/* ---- tape builder (ir.rs:190-366) ---- */
 /* Parses `text` (a clang .ll module holding @_Z1fffff) into out[0..n).  time is
  * the ARGW value written into instruction 3 (ir.rs:348).  Returns ORC_OK, or an
  * error; *n_total receives the untruncated instruction count. */
@@ -96,7 +96,7 @@ float orc_det_pow045(float x);
 float orc_wmin(float a, float b);
 float orc_wmax(float a, float b);
 
-/* ---- rows A/B/E: one cell (compute.wgsl:3622-3996) ---- */
+/* ---- one cell: forward, reverse, time error (compute.wgsl:3622-3996) ---- */
 /* Forward interval op exactly as the `switch (op_top)` of 3642-3766 followed by fix(). */
 orc_iv orc_forward_op(uint32_t op, uint32_t side, orc_iv l, orc_iv r, float literal);
 /* Forward + optional reverse over a tape.  vals/sens: n entries each (may be NULL).
@@ -107,13 +107,13 @@ orc_iv orc_eval_tape(const orc_instr *tape, uint32_t n, const orc_iv args[4], in
 uint32_t orc_fwd_cost(uint32_t op, uint32_t side);
 uint32_t orc_rev_cost(uint32_t op, uint32_t side);
 
-/* ---- row P: pruning (choice-reachability from RET) ---- */
+/* ---- pruning (choice-reachability from RET) ---- */
 /* live[j] = 1 when instruction j is reachable from the output through the
  * MIN/MAX choice edges of `vals`; choice[j] = 1 (left only), 2 (right only),
  * 3 (both) for live MIN/MAX, 0 otherwise. */
 void orc_prune(const orc_instr *tape, uint32_t n, const orc_iv *vals, uint8_t *live, uint8_t *choice);
 
-/* ---- rows C/D/U: whole frames ---- */
+/* ---- whole frames: classification, subdivision, traversal ---- */
 typedef struct orc_state orc_state;
 orc_state *orc_create(const orc_params *p, const orc_instr *tape, uint32_t n, uint32_t voxel_capacity);
 void orc_destroy(orc_state *s);

@@ -111,7 +111,7 @@ class State:
     def __init__(self, tape, width: int = 256, height: int = 256, localsubdiv: int = 8, delta: float = 0.01,
                  point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
                  fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
-                 voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune: bool = True):
+                 voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0):
         if isinstance(tape, IR):
             tape = tape.gpu_instructions
         elif isinstance(tape, (str, bytes)):
@@ -127,7 +127,13 @@ class State:
         p.swept_window = np.float32(swept_window)
         p.voxel_capacity = voxel_capacity
         p.queue_capacity = queue_capacity
-        p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
+        # prune: True = pruned programs wherever the tape allows them, "exact" = only for tapes
+        # whose pruned results are provably identical, False = the full tape in every cell
+        if prune == "exact":
+            p.flags = _lib.TSI_FLAG_PRUNE_EXACT_ONLY
+        else:
+            p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
+        p.prune_capacity = prune_capacity
         self.params = p
         self._ctx = C.c_void_p()
         check(lib().tsi_create(C.byref(p), C.byref(self._ctx)), "tsi_create")

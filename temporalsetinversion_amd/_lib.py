@@ -19,6 +19,7 @@ ERRORS = {
     -10: "TSI_E_BUDGET",
 }
 TSI_FLAG_NO_PRUNE = 1
+TSI_FLAG_PRUNE_EXACT_ONLY = 2
 
 
 class TsiError(RuntimeError):
@@ -44,14 +45,15 @@ class Params(C.Structure):
     _fields_ = [("localsubdiv", C.c_int32), ("delta", C.c_float), ("point_sample", C.c_int32),
                 ("swept_volume", C.c_int32), ("lipschitz_corrected", C.c_int32), ("fastenabled", C.c_int32),
                 ("fast_multiplier", C.c_float), ("swept_window", C.c_float), ("voxel_capacity", C.c_uint64),
-                ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("prune_capacity", C.c_uint32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("gpu_ms", C.c_double), ("eval_ms", C.c_double), ("cell_evals", C.c_uint64),
                 ("rev_evals", C.c_uint64), ("fwd_ops", C.c_uint64), ("rev_ops", C.c_uint64),
                 ("parents", C.c_uint64), ("deallocs", C.c_uint64), ("enqueued", C.c_uint64),
-                ("allocated_nodes", C.c_uint64), ("eval_launches", C.c_uint64), ("pruned_instr", C.c_uint64)]
+                ("allocated_nodes", C.c_uint64), ("eval_launches", C.c_uint64), ("pruned_instr", C.c_uint64),
+                ("prune_launches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

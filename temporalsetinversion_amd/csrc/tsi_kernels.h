@@ -18,7 +18,14 @@ namespace tsi {
 constexpr int kMaxLevels = 24;
 enum : uint32_t { kErrQueue = 1u, kErrPool = 2u };
 enum { kStatCells = 0, kStatRevCells, kStatParents, kStatEnqueued, kStatRevOps, kStatDeallocs, kStatLive,
-       kNumStats = 8 };
+       kStatFwdOps, kStatPruned, kNumStats = 10 };
+
+// Liveness record of a frontier entry (input of its children's pruned program), stored
+// word-major: word w of entry e at rec[w * prune_cap + e], so that a wavefront's consecutive
+// entries write consecutive addresses.  Words: 0 = t-window lo, 1 = t-window hi (of the
+// evaluation that produced the record), then nchunks live words, nchunks left-kept words and
+// nchunks right-kept words, one bit per full-program record.
+constexpr uint32_t kRecHead = 2;
 
 struct Counters {
     uint32_t list_count[kMaxLevels + 2][2];   // frontier sizes per level: [0] forward-only, [1] reverse
@@ -31,6 +38,8 @@ struct Counters {
     uint32_t error;                           // sticky kErr* since reset
     uint32_t root_pending;                    // buf_main has enqueued the root
     uint32_t pad;
+    uint32_t pool_used[kMaxLevels + 2];       // program-pool records taken per level
+    uint32_t pad2[2];
     uint64_t stats[kNumStats];
 };
 
@@ -49,6 +58,16 @@ struct KArgs {
     uint32_t nblocks, list_cap, dlist_cap;
     float epsilon, delta, fast_multiplier, swept_window;
     int32_t swept, point_sample, lipschitz, fastenabled;
+    // pruned programs (per-group dense programs specialised on the parents' liveness)
+    int32_t prune;
+    uint32_t *prune_rec[2][2];  // [level parity][forward-only / reverse], see kRecHead
+    uint32_t prune_cap;         // entries per list that get a record
+    uint32_t nchunks;           // ceil(nfwd / 32)
+    uint32_t *gdesc[2];         // per list: {pool offset, records} for each group of 8 entries
+    uint32_t *choice;           // per wavefront: choice_len words x 64 lanes
+    uint32_t choice_len;
+    FwdIns *pool;               // program pool: forward records then reverse records per group
+    uint32_t pool_cap;          // records
 };
 
 struct TraceArgs {

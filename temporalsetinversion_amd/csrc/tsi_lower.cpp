@@ -141,7 +141,12 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
     if (!tape || n < 5) return TSI_E_ARG;
     if (n > 4096) return TSI_E_TAPE_TOO_LONG;
     std::vector<int> last(n, -1);
-    bool prunable = true;
+    // Pruning class: a decided MIN/MAX stays decided in a child cell when every rule on the
+    // way to its operands is inclusion-monotone in binary32 (child operands inside the
+    // parent's).  Rounded +,-,*,/lit,sqrt,min,max,abs,floor are; the polynomial sin, cos,
+    // exp and log routines are up to ulp-level wiggles; interval division by an interval,
+    // pow, mod and atan2 are not.
+    int prune_class = kPruneExact;
     for (uint32_t j = 4; j < n; ++j) {
         const auto &g = tape[j];
         uint32_t side = side_of(g), op = op_of(g);
@@ -149,22 +154,26 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
         if ((side == 0 || side == 2) && rarg_of(g) >= j) return TSI_E_ARG;
         if (side == 0 || side == 1) last[larg_of(g)] = std::max(last[larg_of(g)], int(j));
         if (side == 0 || side == 2) last[rarg_of(g)] = std::max(last[rarg_of(g)], int(j));
+        if (side == 3) continue;   // lit/lit evaluates to [0, 0] whatever the opcode
         switch (op) {
         case Op::ADD: case Op::SUB: case Op::MUL: case Op::SQR: case Op::SQRT: case Op::NEG: case Op::MIN:
-        case Op::MAX: case Op::ABS: case Op::RET:
+        case Op::MAX: case Op::ABS: case Op::RET: case Op::FLOOR:
             break;
         case Op::DIV:
-            if (side != 1) prunable = false;
+            if (side != 1) prune_class = kPruneOff;
+            break;
+        case Op::SIN: case Op::COS: case Op::EXP: case Op::LOG:
+            if (prune_class == kPruneExact) prune_class = kPruneApprox;
             break;
         default:
-            prunable = false;
+            prune_class = kPruneOff;
         }
     }
     fwd.clear();
     rev.clear();
     L = Lowered{};
     L.n = n;
-    L.prunable = prunable;
+    L.prune_class = prune_class;
 
     // ---- forward slot allocation ----
     std::vector<uint8_t> vslot(n, kNoSlot);
@@ -187,9 +196,14 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
         f.b = (side == 0 || side == 2) ? vslot[b] : uint8_t(0);
         if (f.a == kNoSlot || f.b == kNoSlot) return TSI_E_ARG;
         f.lit = g.literal;
-        f.index = j;
+        f.index = uint16_t(j - 4);
+        f.modes = 0xFFFFu;
+        f.cost = uint8_t(fwd_cost(op, side));
         uint8_t pk = push_kind(Form(f.form));
-        f.npush = pk;
+        f.npush_side = uint8_t(pk | (side << 2));
+        if (op == Op::MIN || op == Op::MAX) {
+            if (f.form == F_MIN_RR || f.form == F_MAX_RR || f.form == F_MIN_RL || f.form == F_MAX_RL) ++L.nminmax;
+        }
         f.push = uint16_t(pos);
         pos += pk == 2 ? 2 : (pk ? 1 : 0);
         // operands whose last use is j die here
@@ -227,8 +241,9 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
         r.side = uint8_t(side);
         r.pop = f.push;
         r.lit = g.literal;
-        r.npush = f.npush;
-        r.cost = uint16_t(rev_cost(op, side));
+        r.npush = f.npush_side & 3u;
+        r.cost = uint8_t(rev_cost(op, side));
+        r.modes = 0xFFFFu;
         r.sa = r.sb = 0;
         if (j == n - 1) {
             r.flags |= R_SJ_ONE;
@@ -253,6 +268,8 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
             slot = sslot[k];
             return true;
         };
+        // lit/lit records do nothing in reverse; they are kept so that reverse record r
+        // belongs to forward record nfwd-1-r (pruned programs select both by index)
         if (side != 3) {
             bool ok = true;
             if (side == 0 || side == 1) ok = ok && touch(a, r.sa, R_FT_A);
@@ -261,8 +278,8 @@ int lower(const tsi_gpu_instruction *tape, uint32_t n, std::vector<FwdIns> &fwd,
             if (side == 0 && a == b && (r.flags & R_FT_A)) r.flags |= R_FT_B;
             if (uses_of(tape[a]) == 1u) r.flags |= R_U1_A;
             if (uses_of(tape[b]) == 1u) r.flags |= R_U1_B;
-            rev.push_back(r);
         }
+        rev.push_back(r);
         if (j == 4) break;
     }
     L.nrev = uint32_t(rev.size());

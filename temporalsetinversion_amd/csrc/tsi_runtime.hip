@@ -39,6 +39,13 @@ struct tsi_ctx {
     size_t stack_bytes = 0;
     FwdIns *fprog = nullptr;
     RevIns *rprog = nullptr;
+    // pruned programs
+    uint32_t *prune_rec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t *gdesc[2] = {nullptr, nullptr};
+    uint32_t *choice = nullptr;
+    FwdIns *pool = nullptr;
+    uint32_t prune_cap = 0, nchunks = 0, choice_len = 0, pool_cap = 0;
+    bool prune = false;
     float4 *rgba = nullptr;
     float *depth = nullptr;
     size_t frame_px = 0;
@@ -101,10 +108,66 @@ KArgs make_args(const tsi_ctx *c) {
     a.point_sample = c->p.point_sample;
     a.lipschitz = c->p.lipschitz_corrected;
     a.fastenabled = c->p.fastenabled;
+    a.prune = c->prune ? 1 : 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) a.prune_rec[i][j] = c->prune_rec[i][j];
+    a.prune_cap = c->prune_cap;
+    a.nchunks = c->nchunks;
+    a.gdesc[0] = c->gdesc[0];
+    a.gdesc[1] = c->gdesc[1];
+    a.choice = c->choice;
+    a.choice_len = c->choice_len;
+    a.pool = c->pool;
+    a.pool_cap = c->pool_cap;
     return a;
 }
 
+void release_prune(tsi_ctx *c) {
+    for (auto &l : c->prune_rec) for (auto &x : l) dfree(x);
+    dfree(c->gdesc[0]); dfree(c->gdesc[1]); dfree(c->choice); dfree(c->pool);
+    c->prune = false;
+    c->prune_cap = c->nchunks = c->choice_len = c->pool_cap = 0;
+}
+
+// Buffers of the pruned programs for a lowered tape: liveness records for prune_cap entries
+// of each frontier list (two level parities), group descriptors, per-wavefront MIN/MAX choice
+// columns, and the program pool (each level reuses it from offset 0).
+int alloc_prune(tsi_ctx *c, const Lowered &L) {
+    release_prune(c);
+    if (L.prune_class == kPruneOff || (c->p.flags & TSI_FLAG_NO_PRUNE)) return TSI_OK;
+    if (L.prune_class == kPruneApprox && (c->p.flags & TSI_FLAG_PRUNE_EXACT_ONLY)) return TSI_OK;
+    uint64_t cap = c->p.prune_capacity ? c->p.prune_capacity : (uint64_t(1) << 19);
+    if (cap > c->list_cap) cap = c->list_cap;
+    cap = (cap + 7) / 8 * 8;
+    const uint32_t nchunks = (L.nfwd + 31) / 32;
+    const uint64_t words = uint64_t(kRecHead + 3 * nchunks) * cap;
+    const uint32_t choice_len = (L.nminmax + 15) / 16 ? (L.nminmax + 15) / 16 : 1;
+    const uint32_t waves = (c->grid_fwd > c->grid_rev ? c->grid_fwd : c->grid_rev) * 4u;
+    // worst case: every group of every list keeps the whole program, twice (fwd + rev)
+    const uint64_t per_group = 2ull * ((L.nfwd + 3) / 4 * 4 + 8);
+    uint64_t pool = 2ull * (cap / 8) * per_group;
+    const uint64_t pool_limit = uint64_t(1) << 27;   // 2 GiB of 16-byte records
+    if (pool > pool_limit) pool = pool_limit;
+    int rc;
+    for (auto &l : c->prune_rec)
+        for (auto &x : l)
+            if ((rc = dmalloc(&x, words))) { release_prune(c); return rc; }
+    if ((rc = dmalloc(&c->gdesc[0], 2 * (c->list_cap / 8 + 1))) ||
+        (rc = dmalloc(&c->gdesc[1], 2 * (c->list_cap / 8 + 1))) ||
+        (rc = dmalloc(&c->choice, size_t(waves) * 64u * choice_len)) || (rc = dmalloc(&c->pool, pool))) {
+        release_prune(c);
+        return rc;
+    }
+    c->prune_cap = uint32_t(cap);
+    c->nchunks = nchunks;
+    c->choice_len = choice_len;
+    c->pool_cap = uint32_t(pool);
+    c->prune = true;
+    return TSI_OK;
+}
+
 void release(tsi_ctx *c) {
+    release_prune(c);
     dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
@@ -246,6 +309,7 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
         if ((rc = dmalloc(&c->stack, need / sizeof(float2)))) return rc;
         c->stack_bytes = need;
     }
+    if ((rc = alloc_prune(c, L))) return rc;
     c->tape.assign(tape, tape + n);
     c->low = L;
     c->have_tape = true;
@@ -293,13 +357,15 @@ int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         stats->eval_ms = ems;
         stats->cell_evals = h.stats[kStatCells];
         stats->rev_evals = h.stats[kStatRevCells];
-        stats->fwd_ops = h.stats[kStatCells] * c->low.fwd_cost;
+        stats->fwd_ops = h.stats[kStatFwdOps];
+        stats->pruned_instr = h.stats[kStatPruned];
         stats->rev_ops = h.stats[kStatRevOps];
         stats->parents = h.stats[kStatParents];
         stats->deallocs = h.stats[kStatDeallocs];
         stats->enqueued = h.stats[kStatEnqueued];
         stats->allocated_nodes = h.stats[kStatLive] * 8u;
         stats->eval_launches = uint64_t(2 * c->p.localsubdiv);
+        stats->prune_launches = c->prune ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
     }
     if (h.frame_error & kErrPool) return TSI_E_POOL_EXHAUSTED;
     if (h.frame_error & kErrQueue) return TSI_E_QUEUE_FULL;

@@ -34,17 +34,22 @@ constexpr uint8_t kNoSlot = 0xFF;
 
 // Forward instruction, 16 bytes.  Operands a/b and destination d are register-file slots.
 // push: stack position (in intervals) where the operand values the reverse rule needs are
-// saved; npush = how many (0, 1 = left, 2 = left and right, 3 = right only).
+// saved; npush_side bits 0-1 = how many (0, 1 = left, 2 = left and right, 3 = right only),
+// bits 2-3 = operand side (0 reg/reg, 1 reg/lit, 2 lit/reg, 3 lit/lit).  index = position of
+// the record in the full program (tape instruction - 4).  modes: the MIN/MAX choice of each
+// of the 8 parents a wavefront evaluates (2 bits each; 0xFFFF in the full program).
 struct FwdIns {
     uint8_t form, a, b, d;
     uint16_t push;
-    uint8_t npush;
-    uint8_t pad;
+    uint8_t npush_side;
+    uint8_t cost;       // algorithmic f32 ops of the forward rule
     float lit;
-    uint32_t index;   // original instruction index (for pruning bitmasks)
+    uint16_t index;
+    uint16_t modes;
 };
 
-// Reverse instruction, 16 bytes (executed for j = n-1 down to 4).
+// Reverse instruction, 16 bytes (executed for j = n-1 down to 4; record r of the full
+// reverse program belongs to forward record nfwd-1-r).
 enum : uint8_t {
     R_SJ_ONE = 1,     // sens(j) = [1,1]: the output (compute.wgsl:3777)
     R_SJ_ZERO = 2,    // sens(j) never accumulated: [0,0]
@@ -59,12 +64,14 @@ struct RevIns {
     uint8_t side;
     uint16_t pop;     // stack position of the saved operands (same as FwdIns::push)
     float lit;
-    uint16_t cost;    // algorithmic f32 ops when not short-circuited
-    uint8_t npush;    // saved operand intervals at `pop` (FwdIns::npush of the same instruction)
-    uint8_t pad;
+    uint8_t cost;     // algorithmic f32 ops when not short-circuited
+    uint8_t npush;    // saved operand intervals at `pop` (FwdIns npush of the same instruction)
+    uint16_t modes;   // as FwdIns::modes
 };
 static_assert(sizeof(FwdIns) == 16, "16-byte forward records");
 static_assert(sizeof(RevIns) == 16, "16-byte reverse records");
+
+enum { kPruneOff = 0, kPruneApprox = 1, kPruneExact = 2 };
 
 // Host-side result of lowering a tape.
 struct Lowered {
@@ -75,7 +82,8 @@ struct Lowered {
     uint32_t arg_slot[4] = {0, 1, 2, 3};
     uint32_t out_slot = 0;          // slot of the field value (instruction n-1)
     uint64_t fwd_cost = 0;          // algorithmic f32 ops of one forward pass
-    int prunable = 0;               // every opcode is inclusion-monotone
+    uint32_t nminmax = 0;           // MIN/MAX records (choice words per lane: ceil(/16))
+    int prune_class = 0;            // kPrune*
 };
 
 }  // namespace tsi

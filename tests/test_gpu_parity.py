@@ -14,11 +14,13 @@ pytestmark = pytest.mark.gpu
 SEQ = [k / 60.0 for k in range(0, 8)] + [0.5, 0.75, 1.0]
 
 
+@pytest.mark.parametrize("prune", [True, False])
 @pytest.mark.parametrize("scene,L", [("sphere", 5), ("spheres_linear", 6), ("spheres", 6)])
-def test_frames_match_oracle(scene, L):
+def test_frames_match_oracle(scene, L, prune):
     tape = tape_of(scene)
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ, localsubdiv=L):
-        print(f"scene={scene} L={L} frame={k} nodes={len(o)} cells={os_.cell_evals}", flush=True)
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ, localsubdiv=L, prune=prune):
+        print(f"scene={scene} L={L} prune={prune} frame={k} nodes={len(o)} cells={os_.cell_evals} "
+              f"pruned={gs['pruned_instr']}", flush=True)
         assert_same_nodes(g, o, f"{scene} frame {k}")
 
 
@@ -34,20 +36,39 @@ def test_param_variants(opts):
 
 def test_stats_match_oracle():
     tape = tape_of("spheres")
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6):
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6, prune=False):
         s = gs
         assert s["cell_evals"] == os_.cell_evals
         assert s["rev_evals"] == os_.rev_evals
         assert s["fwd_ops"] == os_.fwd_ops
         assert s["rev_ops"] == os_.rev_ops
         assert s["enqueued"] == os_.enqueued
+        assert s["pruned_instr"] == 0
 
 
-@pytest.mark.parametrize("scene,L", [("capsules", 4), ("csg", 4)])
-def test_big_tapes_match_oracle(scene, L):
+@pytest.mark.parametrize("scene,L", [("spheres_linear", 6), ("capsules", 5)])
+def test_pruning_prunes(scene, L):
+    """Pruned programs run fewer records than the full tape and the same cells as the oracle."""
     tape = tape_of(scene)
-    for k, g, o, gs, os_, _, _ in run_pair(tape, [0.0, 1 / 60, 2 / 60, 0.3], localsubdiv=L):
-        print(f"scene={scene} frame={k} nodes={len(o)}", flush=True)
+    nfwd = len(tape) - 4
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:4], localsubdiv=L):
+        assert gs["cell_evals"] == os_.cell_evals
+        assert gs["rev_evals"] == os_.rev_evals
+        assert gs["enqueued"] == os_.enqueued
+        assert gs["fwd_ops"] <= os_.fwd_ops
+        print(f"scene={scene} frame={k} cells={gs['cell_evals']} mean records run="
+              f"{nfwd - gs['pruned_instr'] / max(gs['cell_evals'], 1):.1f} of {nfwd}", flush=True)
+        if k == 0:
+            assert gs["pruned_instr"] > 0
+        assert_same_nodes(g, o, f"{scene} frame {k}")
+
+
+@pytest.mark.parametrize("prune", [True, False])
+@pytest.mark.parametrize("scene,L", [("capsules", 4), ("csg", 4)])
+def test_big_tapes_match_oracle(scene, L, prune):
+    tape = tape_of(scene)
+    for k, g, o, gs, os_, _, _ in run_pair(tape, [0.0, 1 / 60, 2 / 60, 0.3], localsubdiv=L, prune=prune):
+        print(f"scene={scene} prune={prune} frame={k} nodes={len(o)} pruned={gs['pruned_instr']}", flush=True)
         assert_same_nodes(g, o, f"{scene} frame {k}")
 
 
