@@ -76,6 +76,11 @@ def main():
     ap.add_argument("--workload", default="capsules64_L9_1024", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", default="compiled", choices=["compiled", "interpreter"],
+                    help="tape evaluation: compiled straight-line kernels (default) or the interpreter")
+    ap.add_argument("--prune", default="on", choices=["on", "off", "exact"],
+                    help="pruned per-group programs: on (default), off (full tape per cell), exact "
+                         "(only for tapes whose pruned results are provably identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs the "
@@ -90,7 +95,9 @@ def main():
     tape = T.tape_from_ll(ll)
     st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
                  point_sample=w["point_sample"], swept_volume=w["swept_volume"],
-                 lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27)
+                 lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27,
+                 prune={"on": True, "off": False, "exact": "exact"}[args.prune], engine=args.engine)
+    info = st.tape_info()
     st.reset()
     frame = 0
 
@@ -104,7 +111,7 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0)
+    tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0, pruned=0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = step()
@@ -116,6 +123,7 @@ def main():
         tot["gpu_ms"] += s["gpu_ms"]
         tot["eval_ms"] += s["eval_ms"]
         tot["launches"] += s["eval_launches"]
+        tot["pruned"] += s["pruned_instr"]
     wall = time.perf_counter() - t0
     st.close()
 
@@ -137,7 +145,10 @@ def main():
         "config": {"workload": args.workload, "tape_instructions": int(len(tape)), "localsubdiv": w["localsubdiv"],
                    "frame": f"{w['width']}x{w['height']}", "delta": 0.01,
                    "lipschitz_corrected": w["lipschitz_corrected"], "swept_volume": w["swept_volume"],
-                   "point_sample": w["point_sample"], "times": "t_k = k/60"},
+                   "point_sample": w["point_sample"], "times": "t_k = k/60", "engine": info["engine"],
+                   "prune": args.prune if info["engine"] == "interpreter" else "n/a"},
+        "tape_compile_s": info["compile_ms"] / 1e3, "tape_compile_cached": bool(info["cache_hit"]),
+        "records_run_per_cell": (len(tape) - 4) - tot["pruned"] / max(tot["cells"], 1),
         "set_inversion_ms_per_frame": tot["gpu_ms"] / args.steps,
         "cell_evals_per_frame": tot["cells"] / args.steps,
         "rev_evals_per_frame": tot["rev"] / args.steps,

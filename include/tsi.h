@@ -29,7 +29,8 @@ enum {
     TSI_E_PARSE = -7,            /* .ll text not understood (src/ir.rs would panic) */
     TSI_E_NO_TAPE = -8,          /* no tape uploaded yet */
     TSI_E_UNSUPPORTED = -9,      /* tape needs more live values than the register file holds */
-    TSI_E_BUDGET = -10           /* served form: request exceeds the HBM admission budget */
+    TSI_E_BUDGET = -10,          /* served form: request exceeds the HBM admission budget */
+    TSI_E_JIT = -11              /* the tape's evaluation kernels did not compile (see tsi_tape_info) */
 };
 
 /* src/ir.rs:88-97 GPUInstruction, 32 bytes: opcode_uses = (op | side<<14) << 16 | uses,
@@ -77,7 +78,11 @@ enum {
      * that add sin, cos, exp or log they are identical unless an interval bound is within a
      * few ulp of a MIN/MAX decision; tapes with any other opcode are never pruned. */
     TSI_FLAG_NO_PRUNE = 1u,
-    TSI_FLAG_PRUNE_EXACT_ONLY = 2u  /* prune only tapes of the first (exact) kind */
+    TSI_FLAG_PRUNE_EXACT_ONLY = 2u, /* prune only tapes of the first (exact) kind */
+    /* Evaluate tapes with the interpreter (programs in constant memory) instead of compiling
+     * each uploaded tape into straight-line evaluation kernels.  Pruning applies to the
+     * interpreter only: compiled kernels always run the full tape. */
+    TSI_FLAG_INTERPRET = 4u
 };
 
 /* Per-frame statistics.  fwd_ops/rev_ops count algorithmic f32 interval operations. */
@@ -150,6 +155,25 @@ int tsi_read_nodes(tsi_ctx *ctx, tsi_node *out, uint64_t cap, uint64_t *count);
 
 /* The uploaded tape as 32-byte records (src/main.rs:822-876 tape dump). */
 int tsi_read_tape(tsi_ctx *ctx, tsi_gpu_instruction *out, uint32_t cap, uint32_t *n);
+
+/* How the uploaded tape is evaluated (no counterpart in the reference, which interprets). */
+typedef struct {
+    uint32_t n;                   /* tape records */
+    uint32_t stack;               /* saved operand intervals per lane (reverse pass) */
+    uint32_t slots;               /* interpreter register-file slots used */
+    uint32_t prune_class;         /* 0 never pruned, 1 pruned up to ulp-level decisions, 2 exact */
+    uint32_t engine;              /* 0 interpreter, 1 compiled straight-line kernels */
+    uint32_t cache_hit;           /* compiled kernels came from the code-object cache */
+    double compile_ms;            /* hiprtc compile (or cache lookup) wall time */
+    char log[1024];               /* compiler log of a failed compile (truncated) */
+} tsi_tape_info;
+int tsi_tape_info_get(tsi_ctx *ctx, tsi_tape_info *out);
+
+/* Compiles a tape's evaluation kernels into the code-object cache without a device or
+ * context (no counterpart in the reference): a later tsi_upload_tape of the same tape then
+ * only loads them.  Cache directory: $TSI_JIT_CACHE, else $XDG_CACHE_HOME/tsi-jit, else
+ * ~/.cache/tsi-jit. */
+int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_info *out);
 
 /* src/ir.rs IR::new + parseLLVMIR + to_gpu: builds the tape from .ll text holding
  * @_Z1fffff.  time is written into instruction 3 (ARGW).  *n receives the full

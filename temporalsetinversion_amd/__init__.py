@@ -46,6 +46,21 @@ def tape_from_ll(text: str | bytes, time: float = 0.0) -> np.ndarray:
     return np.frombuffer(bytes(out), dtype=INSTR_DTYPE, count=cap).copy()
 
 
+def precompile(tape) -> dict:
+    """Compiles a tape's evaluation kernels into the code-object cache (no GPU needed)."""
+    if isinstance(tape, IR):
+        tape = tape.gpu_instructions
+    elif isinstance(tape, (str, bytes)):
+        tape = tape_from_ll(tape)
+    t = np.ascontiguousarray(tape, dtype=INSTR_DTYPE)
+    buf = (GpuInstruction * len(t)).from_buffer_copy(t.tobytes())
+    info = _lib.TapeInfo()
+    rc = lib().tsi_precompile_tape(buf, len(t), C.byref(info))
+    if rc != 0:
+        raise TsiError(rc, "tsi_precompile_tape\n" + info.log.decode(errors="replace"))
+    return {"compile_ms": info.compile_ms, "cache_hit": bool(info.cache_hit), "n": info.n}
+
+
 class IR:
     """Mirror of src/ir.rs `IR`: the tape of one scene."""
 
@@ -111,7 +126,8 @@ class State:
     def __init__(self, tape, width: int = 256, height: int = 256, localsubdiv: int = 8, delta: float = 0.01,
                  point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
                  fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
-                 voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0):
+                 voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0,
+                 engine: str = "compiled"):
         if isinstance(tape, IR):
             tape = tape.gpu_instructions
         elif isinstance(tape, (str, bytes)):
@@ -133,12 +149,23 @@ class State:
             p.flags = _lib.TSI_FLAG_PRUNE_EXACT_ONLY
         else:
             p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
+        # engine: "compiled" = the tape compiled into straight-line kernels at upload (full tape
+        # per cell), "interpreter" = programs in constant memory, pruned per frontier group
+        if engine not in ("compiled", "interpreter"):
+            raise ValueError(f"engine must be 'compiled' or 'interpreter', not {engine!r}")
+        if engine == "interpreter":
+            p.flags |= _lib.TSI_FLAG_INTERPRET
         p.prune_capacity = prune_capacity
         self.params = p
         self._ctx = C.c_void_p()
         check(lib().tsi_create(C.byref(p), C.byref(self._ctx)), "tsi_create")
         buf = (GpuInstruction * len(self.tape)).from_buffer_copy(self.tape.tobytes())
-        check(lib().tsi_upload_tape(self._ctx, buf, len(self.tape)), "tsi_upload_tape")
+        rc = lib().tsi_upload_tape(self._ctx, buf, len(self.tape))
+        if rc != 0:
+            log = self.tape_info()["log"] if rc == -11 else ""
+            lib().tsi_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+            raise TsiError(rc, "tsi_upload_tape" + (f"\n{log}" if log else ""))
         self.uniforms = Uniforms()
         self.uniforms.resolution[0] = width
         self.uniforms.resolution[1] = height
@@ -198,6 +225,14 @@ class State:
         out = np.zeros(n.value, NODE_DTYPE)
         check(lib().tsi_read_nodes(self._ctx, out.ctypes.data, n.value, C.byref(n)), "tsi_read_nodes")
         return out
+
+    def tape_info(self) -> dict:
+        info = _lib.TapeInfo()
+        check(lib().tsi_tape_info_get(self._ctx, C.byref(info)), "tsi_tape_info_get")
+        d = {k: getattr(info, k) for k, _ in info._fields_}
+        d["log"] = info.log.decode(errors="replace")
+        d["engine"] = ("interpreter", "compiled")[info.engine]
+        return d
 
     def read_tape(self) -> np.ndarray:
         n = C.c_uint32(0)

@@ -11,15 +11,19 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSI_LIB", os.path.join(HERE, "_build", "libtsi.so"))
+# compiled tape kernels are cached next to the library unless the caller chose a directory
+JIT_CACHE = os.path.join(HERE, "_build", "jit-cache")
+os.environ.setdefault("TSI_JIT_CACHE", JIT_CACHE)
 
 TSI_OK = 0
 ERRORS = {
     -1: "TSI_E_OOM", -2: "TSI_E_QUEUE_FULL", -3: "TSI_E_POOL_EXHAUSTED", -4: "TSI_E_TAPE_TOO_LONG",
     -5: "TSI_E_HIP", -6: "TSI_E_ARG", -7: "TSI_E_PARSE", -8: "TSI_E_NO_TAPE", -9: "TSI_E_UNSUPPORTED",
-    -10: "TSI_E_BUDGET",
+    -10: "TSI_E_BUDGET", -11: "TSI_E_JIT",
 }
 TSI_FLAG_NO_PRUNE = 1
 TSI_FLAG_PRUNE_EXACT_ONLY = 2
+TSI_FLAG_INTERPRET = 4
 
 
 class TsiError(RuntimeError):
@@ -59,6 +63,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TapeInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("stack", C.c_uint32), ("slots", C.c_uint32), ("prune_class", C.c_uint32),
+                ("engine", C.c_uint32), ("cache_hit", C.c_uint32), ("compile_ms", C.c_double),
+                ("log", C.c_char * 1024)]
+
+
 class Node(C.Structure):
     _fields_ = [("depth", C.c_uint32), ("i", C.c_uint32), ("j", C.c_uint32), ("k", C.c_uint32),
                 ("f", C.c_float), ("t", C.c_float), ("time_err", C.c_float), ("fast", C.c_float),
@@ -82,6 +92,8 @@ SIGNATURES = {
     "tsi_tape_from_ll": (C.c_int, [C.c_char_p, C.c_size_t, C.c_float, C.POINTER(GpuInstruction), C.c_uint32,
                                    C.POINTER(C.c_uint32)]),
     "tsi_camera_zdist": (C.c_float, [C.c_uint32]),
+    "tsi_tape_info_get": (C.c_int, [C.c_void_p, C.POINTER(TapeInfo)]),
+    "tsi_precompile_tape": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.POINTER(TapeInfo)]),
     "tsi_strerror": (C.c_char_p, [C.c_int]),
 }
 

@@ -1,0 +1,260 @@
+// tsi_eval.h -- one level of the set inversion for one frontier list (compute.wgsl:3513-4056
+// restated level-synchronously): allocation of the children of new parents, child boxes and
+// t-windows, tape evaluation through an Engine, time error, fast flag, classification and
+// compaction of the next frontier.  A 64-lane wavefront takes 8 parents x 8 child octants
+// (lane = 8 * parent slot + octant, octant = x + 2y + 4z as compute.wgsl:3576).
+//
+// Engine (evaluation of the tape for the wavefront's 64 cells):
+//   Engine(a, wave, lane)
+//   select(a, which, g, pruned_ok)  program for group g (pruned_ok: the group's cells lie
+//                                   inside their parents)
+//   forward<PUSH>(a, x, y, z, t, stk, field, Rlo, Rhi) -> algorithmic f32 ops; PUSH saves the
+//                                   operands the reverse rules need on the wavefront's stack
+//                                   (stk: uniform base, row r of lane l at stk[64 r + l])
+//   reverse(a, stk, sens[4], Rlo, Rhi) -> algorithmic f32 ops; sensitivities of x, y, z, t
+//   (Rlo/Rhi: a register file the engine may use between the two calls)
+//   records()                      records run per forward pass
+//   liveness(a, rec)               the cell's liveness record (kPrune engines)
+#pragma once
+#include "tsi_wave.h"
+
+namespace tsi {
+typedef float Slots[kSlots];
+}
+
+namespace tsi {
+
+
+template <bool REV, class Engine>
+TSI_DI void eval_level(const KArgs &a, int level, float time) {
+    const int which = REV ? 1 : 0;
+    Counters *cnt = a.cnt;
+    // a list that overflowed (flagged by wave_append) holds list_cap valid entries
+    const uint32_t count = min(cnt->list_count[level][which], a.list_cap);
+    if (count == 0) return;
+    const uint32_t *list = a.lists[level & 1][which];
+    uint32_t *next_f = a.lists[(level + 1) & 1][0];
+    uint32_t *next_r = a.lists[(level + 1) & 1][1];
+    const uint32_t lane = lane_id();
+    const uint32_t ps = lane >> 3, o = lane & 7u;
+    const uint32_t lx = o & 1u, ly = (o >> 1) & 1u, lz = o >> 2;
+    // wave index made visibly uniform: the grid-stride loop and everything inside it then
+    // compile as scalar control flow
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    // this wavefront's saved-operand stack (uniform base; lane l uses column l of each row)
+    float2 *stk = REV ? a.stack + size_t(wave) * a.stack_len * 64 : nullptr;
+    const float eps = a.epsilon;
+    const bool prune = Engine::kPrune && a.prune != 0;
+    // this level's parents' liveness records (written by the previous level) and the next
+    // level's (written here)
+    const uint32_t *prec_in = a.prune_rec[level & 1][which];
+    uint32_t *prec_f = a.prune_rec[(level + 1) & 1][0];
+    uint32_t *prec_r = a.prune_rec[(level + 1) & 1][1];
+    Engine E(a, wave, lane);
+    // register file of engines that keep values in a uniformly indexed array (a kernel-local
+    // array, so that it is promoted to VGPRs; unused by straight-line engines)
+    Slots Rlo, Rhi;
+
+    uint64_t n_cells = 0, n_rev = 0, n_par = 0, n_enq = 0, rev_ops = 0, fwd_ops = 0, n_pruned = 0;
+
+    for (uint32_t g = wave; g * 8u < count; g += nwaves) {
+        const uint32_t pidx = g * 8u + ps;
+        const bool active = pidx < count;
+        const uint32_t P = active ? list[pidx] : 0u;
+        const float4 pst = a.st[P];
+        const uint32_t pmeta = meta_of(pst);
+        const bool isnew = active && (pmeta & kChildMask) == 0;
+        const bool pfast = (pmeta & kFastBit) != 0;
+
+        // ---- allocate child blocks for new parents: one atomic per wavefront ----
+        uint32_t blk = (pmeta & kChildMask) - 1u;
+        {
+            const uint64_t req = ballot(isnew && o == 0);
+            if (req) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&cnt->alloc_tickets, popc64(req));
+                base = bcast(base, 0);
+                uint32_t tk = base + rank_in(req);
+                tk = bcast(tk, int(lane & ~7u));
+                if (isnew) {
+                    const uint32_t F = cnt->free_top;
+                    if (tk < F) blk = a.free_stack[F - 1u - tk];
+                    else blk = 1u + cnt->bump + (tk - F);
+                }
+            }
+        }
+        bool ok = active;
+        if (isnew && blk >= a.nblocks) {
+            ok = false;
+            if (o == 0) atomicOr(&cnt->frame_error, kErrPool);
+        }
+        if (isnew && ok && o == 0) a.st[P].w = __uint_as_float((pmeta & kFastBit) | (blk + 1u));
+        n_par += (active && o == 0) ? 1 : 0;
+
+        const uint32_t C = ok ? blk * 8u + o : 0u;
+        float4 cst = make_float4(1.0f, 0.0f, 0.0f, 0.0f);   // mallocV (compute.wgsl:3437-3449)
+        if (ok && !isnew) cst = a.st[C];
+        uint32_t cmeta = meta_of(cst);
+
+        // ---- child box, t-window, point sampling (compute.wgsl:3582-3626) ----
+        const float4 plo = a.blo[P], phi = a.bhi[P];
+        const float dx = (phi.x - plo.x) / 2.0f, dy = (phi.y - plo.y) / 2.0f, dz = (phi.z - plo.z) / 2.0f;
+        const float width = wmax(dx, wmax(dy, dz));
+        Iv cx = I(plo.x + dx * float(lx), plo.x + dx * float(1u + lx));
+        Iv cy = I(plo.y + dy * float(ly), plo.y + dy * float(1u + ly));
+        Iv cz = I(plo.z + dz * float(lz), plo.z + dz * float(1u + lz));
+        if (ok) {
+            a.blo[C] = make_float4(cx.lo, cy.lo, cz.lo, 0.0f);
+            a.bhi[C] = make_float4(cx.hi, cy.hi, cz.hi, 0.0f);
+        }
+        Iv ct;
+        if (isnew || pfast || width <= eps) {
+            ct = a.swept ? I(time - a.swept_window, time) : I_FROM(time);
+            cmeta &= ~kFastBit;
+        } else {
+            ct = I(cst.y, time);
+        }
+        float leaf_w = width;
+        if (a.point_sample && width <= eps) {
+            cx = I_FROM(MID(cx));
+            cy = I_FROM(MID(cy));
+            cz = I_FROM(MID(cz));
+            leaf_w = 0.0f;
+        }
+        const bool guard = ok && (isnew || cst.x >= 0.0f || (cmeta & kChildMask) == 0);
+        const Iv targ = I_FROM_UNORDERED(ct.lo, ct.hi);
+        const bool td = ct.lo != ct.hi;
+
+        // ---- program: the group's pruned program when every evaluated child lies inside its
+        //      parent (box and t-window of the evaluation that produced the liveness record),
+        //      else the full tape ----
+        bool pruned_ok = false;
+        if (prune && level > 0) {
+            const bool has = active && pidx < a.prune_cap;
+            const float wlo = has ? __uint_as_float(prec_in[pidx]) : 0.0f;
+            const float whi = has ? __uint_as_float(prec_in[a.prune_cap + pidx]) : 0.0f;
+            const bool inside = targ.lo >= wlo && targ.hi <= whi && cx.lo >= plo.x && cx.hi <= phi.x &&
+                                cy.lo >= plo.y && cy.hi <= phi.y && cz.lo >= plo.z && cz.hi <= phi.z;
+            pruned_ok = ballot(guard && !inside) == 0;
+        }
+        E.select(a, which, g, pruned_ok);
+
+        // ---- tape evaluation (wave-uniform) ----
+        Iv field = I_FROM(0.0f);
+        const uint64_t need_eval = ballot(guard);
+        if (need_eval) {
+            const uint32_t fc = E.template forward<REV>(a, cx, cy, cz, targ, stk, field, Rlo, Rhi);
+            if (guard) {
+                fwd_ops += fc;
+                n_pruned += a.nfwd - E.records();
+            }
+        }
+        Iv df = I_FROM(0.0f), df_dt = I_FROM(0.0f);
+        float4 gradv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float terr = cst.z;
+        if (REV) {
+            const uint64_t need_rev = ballot(guard && td);
+            if (need_rev) {
+                Iv sens[4];
+                const uint32_t ops = E.reverse(a, stk, sens, Rlo, Rhi);
+                if (guard && td) {
+                    rev_ops += ops;
+                    const Iv s0 = sens[0], s1 = sens[1], s2 = sens[2];
+                    // the sign of a zero sensitivity depends on which pruned instructions ran;
+                    // dx/dy/dz are stored with +0 for any zero
+                    gradv = make_float4(MID(s0) + 0.0f, MID(s1) + 0.0f, MID(s2) + 0.0f, 0.0f);   // 3967-3969
+                    df_dt = sens[3];                                             // 3972
+                    const float dt = DIAMETER(targ);
+                    df = MUL_IF(df_dt, dt);
+                    const Iv W = MUL_IF(I(-1.0f, 1.0f), DIAMETER(field));
+                    df = CAP_II(df, W);
+                    if (a.lipschitz && ABS_I(df).hi != 0.0f) {                    // 3980-3992
+                        const float lb =
+                            ABS_I(SQRT_I(ADD_II(SQR_I(s0), ADD_II(SQR_I(s1), SQR_I(s2))))).lo;
+                        if (!is_nan_w(lb)) df = DIV_II(df, I_FROM(lb));
+                    }
+                    const float inf = __uint_as_float(0x7f800000u);
+                    if (df.lo == -inf || df.hi == inf || is_nan_w(df.lo) || is_nan_w(df.hi)) df = I_FROM(a.delta);
+                    terr = terr + wmin(wmax(ABS_I(df).hi, 0.0f), a.delta);       // 3996
+                }
+            }
+        }
+
+        // ---- fast flag and classification (compute.wgsl:4003-4045) ----
+        float f = cst.x;
+        bool enq = false, drop_children = false;
+        if (guard) {
+            if (!a.swept && a.fastenabled) {
+                if (ABS_I(df).hi >= a.delta * a.fast_multiplier) cmeta |= kFastBit;
+                else cmeta &= ~kFastBit;
+            }
+            if (field.lo <= 0.0f && 0.0f <= field.hi) {
+                f = 0.0f;
+                if (leaf_w > eps) {
+                    const float inf = __uint_as_float(0x7f800000u);
+                    if (isnew || is_nan_w(df_dt.lo) || is_nan_w(df_dt.hi) || fabsf(df_dt.lo) == inf ||
+                        fabsf(df_dt.hi) == inf || pfast || ct.lo == ct.hi || terr >= a.delta ||
+                        (cmeta & kChildMask) == 0) {
+                        terr = 0.0f;
+                        enq = true;
+                    }
+                }
+            } else {
+                if (field.hi <= 0.0f) f = -1.0f;
+                else if (!a.swept) f = 1.0f;
+                if (!a.swept && (cmeta & kChildMask) != 0) drop_children = true;
+            }
+        }
+        const uint32_t old_child = cmeta & kChildMask;
+        if (drop_children) cmeta &= ~kChildMask;
+        if (ok) a.st[C] = make_float4(f, time, terr, __uint_as_float(cmeta));
+        // dx/dy/dz: written by the reverse pass; zeroed when the node is allocated, so a
+        // reused node never shows a previous occupant's values.
+        if (ok && (isnew || (REV && guard && td))) a.grad[C] = gradv;
+
+        // ---- compaction of the outputs ----
+        const bool next_fwd_only =
+            !a.swept && ((cmeta & kChildMask) == 0 || (cmeta & kFastBit) || width / 2.0f <= eps);
+        const uint32_t at_f = wave_append(enq && next_fwd_only, C, next_f, &cnt->list_count[level + 1][0],
+                                          a.list_cap, &cnt->frame_error, kErrQueue);
+        const uint32_t at_r = wave_append(enq && !next_fwd_only, C, next_r, &cnt->list_count[level + 1][1],
+                                          a.list_cap, &cnt->frame_error, kErrQueue);
+        wave_append(drop_children, old_child - 1u, a.dlists[0], &cnt->dlist_count[0], a.dlist_cap,
+                    &cnt->frame_error, kErrQueue);
+
+        // ---- liveness of the enqueued cells, for their children's pruned programs ----
+        if (prune && ballot(enq) != 0) {
+            uint32_t *rec = nullptr;
+            if (at_f < a.prune_cap) rec = prec_f + at_f;
+            if (at_r < a.prune_cap) rec = prec_r + at_r;
+            if (rec) {
+                rec[0] = __float_as_uint(targ.lo);
+                rec[a.prune_cap] = __float_as_uint(targ.hi);
+            }
+            E.liveness(a, rec);
+        }
+        n_cells += guard ? 1 : 0;
+        n_rev += (guard && td) ? 1 : 0;
+        n_enq += enq ? 1 : 0;
+    }
+    // ---- statistics: one atomic per wavefront ----
+    n_cells = wave_sum64(n_cells);
+    n_rev = wave_sum64(n_rev);
+    n_par = wave_sum64(n_par);
+    n_enq = wave_sum64(n_enq);
+    rev_ops = wave_sum64(rev_ops);
+    fwd_ops = wave_sum64(fwd_ops);
+    n_pruned = wave_sum64(n_pruned);
+    if (lane == 0) {
+        if (n_cells) atomicAdd((unsigned long long *)&cnt->stats[kStatCells], (unsigned long long)n_cells);
+        if (n_rev) atomicAdd((unsigned long long *)&cnt->stats[kStatRevCells], (unsigned long long)n_rev);
+        if (n_par) atomicAdd((unsigned long long *)&cnt->stats[kStatParents], (unsigned long long)n_par);
+        if (n_enq) atomicAdd((unsigned long long *)&cnt->stats[kStatEnqueued], (unsigned long long)n_enq);
+        if (rev_ops) atomicAdd((unsigned long long *)&cnt->stats[kStatRevOps], (unsigned long long)rev_ops);
+        if (fwd_ops) atomicAdd((unsigned long long *)&cnt->stats[kStatFwdOps], (unsigned long long)fwd_ops);
+        if (n_pruned) atomicAdd((unsigned long long *)&cnt->stats[kStatPruned], (unsigned long long)n_pruned);
+    }
+}
+
+}  // namespace tsi

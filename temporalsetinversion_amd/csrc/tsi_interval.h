@@ -15,8 +15,14 @@
 // the per-lane code free of divergent branches lets the compiler emit that dispatch as plain
 // scalar branches instead of exec-mask flow blocks.
 #pragma once
+#ifdef __HIPCC_RTC__   // hiprtc provides the runtime declarations; fixed-width types live here
+using __hip_internal::int32_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace tsi {
 

@@ -8,8 +8,10 @@
 // Children of a node are one block of 8 consecutive nodes (child index x + 2y + 4z, the
 // numbering of compute.wgsl:3576 and 4295).  Node 0 is the root; block 0 holds only it.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "tsi_tape.h"
 
@@ -83,11 +85,15 @@ struct FrameLaunch {
     int levels;
     float time;
     uint32_t grid_fwd, grid_rev, grid_misc;
+    void *tape_eval[2];   // compiled level kernels (hipFunction_t) of the tape, or null: k_eval
 };
 
+#ifndef __HIPCC_RTC__
 int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1);
 int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
 int launch_enqueue_root(const KArgs &a, hipStream_t s);
 int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
+
+#endif
 
 }  // namespace tsi

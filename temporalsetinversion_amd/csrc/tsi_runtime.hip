@@ -8,9 +8,12 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "tsi.h"
+#include "tsi_jit.h"
 #include "tsi_kernels.h"
 #include "tsi_tape.h"
 
@@ -46,6 +49,10 @@ struct tsi_ctx {
     FwdIns *pool = nullptr;
     uint32_t prune_cap = 0, nchunks = 0, choice_len = 0, pool_cap = 0;
     bool prune = false;
+    // compiled per-tape evaluation kernels
+    hipModule_t mod[2] = {nullptr, nullptr};
+    hipFunction_t eval_fn[2] = {nullptr, nullptr};   // forward-only list, reverse list
+    tsi_tape_info info{};
     float4 *rgba = nullptr;
     float *depth = nullptr;
     size_t frame_px = 0;
@@ -94,7 +101,7 @@ KArgs make_args(const tsi_ctx *c) {
     a.rprog = c->rprog;
     a.nfwd = c->low.nfwd;
     a.nrev = c->low.nrev;
-    a.stack_len = c->low.stack;
+    a.stack_len = c->low.stack + 1;
     a.out_slot = c->low.out_slot;
     for (int k = 0; k < 4; ++k) a.arg_slot[k] = c->low.arg_slot[k];
     a.nblocks = c->nblocks;
@@ -135,6 +142,7 @@ void release_prune(tsi_ctx *c) {
 int alloc_prune(tsi_ctx *c, const Lowered &L) {
     release_prune(c);
     if (L.prune_class == kPruneOff || (c->p.flags & TSI_FLAG_NO_PRUNE)) return TSI_OK;
+    if (!(c->p.flags & TSI_FLAG_INTERPRET)) return TSI_OK;   // compiled kernels run the full tape
     if (L.prune_class == kPruneApprox && (c->p.flags & TSI_FLAG_PRUNE_EXACT_ONLY)) return TSI_OK;
     uint64_t cap = c->p.prune_capacity ? c->p.prune_capacity : (uint64_t(1) << 19);
     if (cap > c->list_cap) cap = c->list_cap;
@@ -166,7 +174,54 @@ int alloc_prune(tsi_ctx *c, const Lowered &L) {
     return TSI_OK;
 }
 
+void release_jit(tsi_ctx *c) {
+    for (int k = 0; k < 2; ++k) {
+        if (c->mod[k]) hipModuleUnload(c->mod[k]);
+        c->mod[k] = nullptr;
+        c->eval_fn[k] = nullptr;
+    }
+}
+
+// Compiles (or finds cached) the tape's two level kernels and loads them on this device.
+int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
+             const std::vector<RevIns> &rev, const Lowered &L) {
+    release_jit(c);
+    std::string src[2], log[2];
+    std::vector<char> code[2];
+    bool hit[2] = {false, false};
+    double ms[2] = {0.0, 0.0};
+    int rc[2] = {TSI_OK, TSI_OK};
+    try {
+        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1);
+        // the two kernels compile in parallel
+        std::thread t([&] { rc[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
+        rc[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
+        t.join();
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    c->info.compile_ms = ms[0] > ms[1] ? ms[0] : ms[1];
+    c->info.cache_hit = hit[0] && hit[1];
+    for (int k = 0; k < 2; ++k) {
+        if (rc[k] != TSI_OK) {
+            std::strncpy(c->info.log, log[k].c_str(), sizeof c->info.log - 1);
+            c->info.log[sizeof c->info.log - 1] = 0;
+            return rc[k];
+        }
+    }
+    static const char *const names[2] = {"tsi_tape_eval_fwd", "tsi_tape_eval_rev"};
+    for (int k = 0; k < 2; ++k) {
+        if (hipModuleLoadData(&c->mod[k], code[k].data()) != hipSuccess ||
+            hipModuleGetFunction(&c->eval_fn[k], c->mod[k], names[k]) != hipSuccess) {
+            release_jit(c);
+            return TSI_E_HIP;
+        }
+    }
+    return TSI_OK;
+}
+
 void release(tsi_ctx *c) {
+    release_jit(c);
     release_prune(c);
     dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
@@ -209,6 +264,7 @@ const char *tsi_strerror(int code) {
     case TSI_E_NO_TAPE: return "no tape uploaded";
     case TSI_E_UNSUPPORTED: return "tape needs more live values than the register file holds";
     case TSI_E_BUDGET: return "request exceeds the admission budget";
+    case TSI_E_JIT: return "the tape's evaluation kernels did not compile";
     default: return "unknown error";
     }
 }
@@ -295,14 +351,15 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     HIP_OK(hipStreamSynchronize(c->stream));
     dfree(c->fprog);
     dfree(c->rprog);
-    // zero padding records: the kernels fetch programs in batches of four records, one batch
-    // ahead of execution
-    fwd.resize((fwd.size() + 3) / 4 * 4 + 8, FwdIns{});
-    rev.resize((rev.size() + 3) / 4 * 4 + 8, RevIns{});
+    // zero padding records: the interpreter fetches program records ahead of execution
+    fwd.resize(program_stride(uint32_t(fwd.size())), FwdIns{});
+    rev.resize(program_stride(uint32_t(rev.size())), RevIns{});
     if ((rc = dmalloc(&c->fprog, fwd.size())) || (rc = dmalloc(&c->rprog, rev.size()))) return rc;
     HIP_OK(hipMemcpy(c->fprog, fwd.data(), fwd.size() * sizeof(FwdIns), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->rprog, rev.data(), rev.size() * sizeof(RevIns), hipMemcpyHostToDevice));
-    size_t need = size_t(c->grid_rev) * 4u * 64u * size_t(L.stack ? L.stack : 1) * sizeof(float2);
+    // per wavefront: L.stack rows of 64 lanes, plus one spare row (the reverse pass loads two
+    // intervals per record whatever the record saved)
+    size_t need = size_t(c->grid_rev) * 4u * 64u * size_t(L.stack + 1) * sizeof(float2);
     if (need > c->stack_bytes) {
         dfree(c->stack);
         c->stack_bytes = 0;
@@ -310,6 +367,22 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
         c->stack_bytes = need;
     }
     if ((rc = alloc_prune(c, L))) return rc;
+    std::memset(&c->info, 0, sizeof c->info);
+    c->info.n = n;
+    c->info.stack = L.stack;
+    c->info.slots = L.slots;
+    c->info.prune_class = uint32_t(L.prune_class);
+    if (!(c->p.flags & TSI_FLAG_INTERPRET)) {
+        fwd.resize(L.nfwd);
+        rev.resize(L.nrev);
+        if ((rc = load_jit(c, tape, n, fwd, rev, L))) {
+            c->have_tape = false;
+            return rc;
+        }
+        c->info.engine = 1;
+    } else {
+        release_jit(c);
+    }
     c->tape.assign(tape, tape + n);
     c->low = L;
     c->have_tape = true;
@@ -340,7 +413,8 @@ int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
     if (!c) return TSI_E_ARG;
     if (!c->have_tape) return TSI_E_NO_TAPE;
     KArgs a = make_args(c);
-    FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc};
+    FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
+                   {(void *)c->eval_fn[0], (void *)c->eval_fn[1]}};
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
     if (launch_frame(a, fl, c->stream, c->ev[2], c->ev[3]) != 0) return TSI_E_HIP;
     HIP_OK(hipEventRecord(c->ev[1], c->stream));
@@ -420,6 +494,53 @@ int tsi_read_tape(tsi_ctx *c, tsi_gpu_instruction *out, uint32_t cap, uint32_t *
         uint32_t m = uint32_t(c->tape.size()) < cap ? uint32_t(c->tape.size()) : cap;
         std::memcpy(out, c->tape.data(), m * sizeof(tsi_gpu_instruction));
     }
+    return TSI_OK;
+}
+
+int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_info *out) {
+    if (!tape) return TSI_E_ARG;
+    if (n > 4096) return TSI_E_TAPE_TOO_LONG;
+    std::vector<FwdIns> fwd;
+    std::vector<RevIns> rev;
+    Lowered L;
+    tsi_tape_info info{};
+    int rc;
+    try {
+        rc = lower(tape, n, fwd, rev, L);
+        if (rc == TSI_OK) {
+            info.n = n;
+            info.stack = L.stack;
+            info.slots = L.slots;
+            info.prune_class = uint32_t(L.prune_class);
+            info.engine = 1;
+            std::string src[2], log[2];
+            std::vector<char> code[2];
+            bool hit[2] = {false, false};
+            double ms[2] = {0.0, 0.0};
+            int r[2] = {TSI_OK, TSI_OK};
+            for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1);
+            std::thread t([&] { r[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
+            r[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
+            t.join();
+            info.compile_ms = ms[0] > ms[1] ? ms[0] : ms[1];
+            info.cache_hit = hit[0] && hit[1];
+            for (int k = 0; k < 2 && rc == TSI_OK; ++k) {
+                if (r[k] != TSI_OK) {
+                    rc = r[k];
+                    std::strncpy(info.log, log[k].c_str(), sizeof info.log - 1);
+                }
+            }
+        }
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    if (out) *out = info;
+    return rc;
+}
+
+int tsi_tape_info_get(tsi_ctx *c, tsi_tape_info *out) {
+    if (!c || !out) return TSI_E_ARG;
+    *out = c->info;
     return TSI_OK;
 }
 

@@ -1,7 +1,19 @@
 // tsi_tape.h -- opcode numbering and the lowered (device) tape format, shared by the host
 // lowering pass and the HIP kernels.
 #pragma once
+#ifdef __HIPCC_RTC__
+using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#else
 #include <stdint.h>
+#endif
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#define TSI_HD __host__ __device__
+#else
+#define TSI_HD
+#endif
 
 namespace tsi {
 
@@ -72,6 +84,10 @@ static_assert(sizeof(FwdIns) == 16, "16-byte forward records");
 static_assert(sizeof(RevIns) == 16, "16-byte reverse records");
 
 enum { kPruneOff = 0, kPruneApprox = 1, kPruneExact = 2 };
+
+// Records a program of m records occupies, zero padding included: the interpreter fetches
+// 16-record groups ahead of execution (4 batches of 4 past the start of its last pair).
+inline TSI_HD uint32_t program_stride(uint32_t m) { return (m + 7u) / 8u * 8u + 16u; }
 
 // Host-side result of lowering a tape.
 struct Lowered {

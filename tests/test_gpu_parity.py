@@ -13,30 +13,38 @@ pytestmark = pytest.mark.gpu
 
 SEQ = [k / 60.0 for k in range(0, 8)] + [0.5, 0.75, 1.0]
 
+# evaluation engines: the tape compiled into straight-line kernels (default), and the
+# interpreter with and without pruned per-group programs
+ENGINES = {"compiled": dict(engine="compiled"),
+           "interp-pruned": dict(engine="interpreter", prune=True),
+           "interp-full": dict(engine="interpreter", prune=False)}
 
-@pytest.mark.parametrize("prune", [True, False])
+
+@pytest.mark.parametrize("engine", sorted(ENGINES))
 @pytest.mark.parametrize("scene,L", [("sphere", 5), ("spheres_linear", 6), ("spheres", 6)])
-def test_frames_match_oracle(scene, L, prune):
+def test_frames_match_oracle(scene, L, engine):
     tape = tape_of(scene)
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ, localsubdiv=L, prune=prune):
-        print(f"scene={scene} L={L} prune={prune} frame={k} nodes={len(o)} cells={os_.cell_evals} "
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ, localsubdiv=L, **ENGINES[engine]):
+        print(f"scene={scene} L={L} engine={engine} frame={k} nodes={len(o)} cells={os_.cell_evals} "
               f"pruned={gs['pruned_instr']}", flush=True)
         assert_same_nodes(g, o, f"{scene} frame {k}")
 
 
+@pytest.mark.parametrize("engine", ["compiled", "interp-pruned"])
 @pytest.mark.parametrize("opts", [dict(swept_volume=True), dict(lipschitz_corrected=True),
                                   dict(point_sample=False), dict(fastenabled=False),
                                   dict(delta=0.05, fast_multiplier=0.5)])
-def test_param_variants(opts):
+def test_param_variants(opts, engine):
     tape = tape_of("spheres")
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:6], localsubdiv=5, **opts):
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:6], localsubdiv=5, **opts, **ENGINES[engine]):
         print(f"opts={opts} frame={k} nodes={len(o)}", flush=True)
         assert_same_nodes(g, o, f"{opts} frame {k}")
 
 
-def test_stats_match_oracle():
+@pytest.mark.parametrize("engine", ["compiled", "interp-full"])
+def test_stats_match_oracle(engine):
     tape = tape_of("spheres")
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6, prune=False):
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6, **ENGINES[engine]):
         s = gs
         assert s["cell_evals"] == os_.cell_evals
         assert s["rev_evals"] == os_.rev_evals
@@ -51,7 +59,7 @@ def test_pruning_prunes(scene, L):
     """Pruned programs run fewer records than the full tape and the same cells as the oracle."""
     tape = tape_of(scene)
     nfwd = len(tape) - 4
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:4], localsubdiv=L):
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:4], localsubdiv=L, **ENGINES["interp-pruned"]):
         assert gs["cell_evals"] == os_.cell_evals
         assert gs["rev_evals"] == os_.rev_evals
         assert gs["enqueued"] == os_.enqueued
@@ -63,12 +71,12 @@ def test_pruning_prunes(scene, L):
         assert_same_nodes(g, o, f"{scene} frame {k}")
 
 
-@pytest.mark.parametrize("prune", [True, False])
+@pytest.mark.parametrize("engine", sorted(ENGINES))
 @pytest.mark.parametrize("scene,L", [("capsules", 4), ("csg", 4)])
-def test_big_tapes_match_oracle(scene, L, prune):
+def test_big_tapes_match_oracle(scene, L, engine):
     tape = tape_of(scene)
-    for k, g, o, gs, os_, _, _ in run_pair(tape, [0.0, 1 / 60, 2 / 60, 0.3], localsubdiv=L, prune=prune):
-        print(f"scene={scene} prune={prune} frame={k} nodes={len(o)} pruned={gs['pruned_instr']}", flush=True)
+    for k, g, o, gs, os_, _, _ in run_pair(tape, [0.0, 1 / 60, 2 / 60, 0.3], localsubdiv=L, **ENGINES[engine]):
+        print(f"scene={scene} engine={engine} frame={k} nodes={len(o)} pruned={gs['pruned_instr']}", flush=True)
         assert_same_nodes(g, o, f"{scene} frame {k}")
 
 
@@ -82,3 +90,13 @@ def test_trace_matches_oracle():
         assert np.array_equal(grgba.view(np.uint32), orgba.view(np.uint32)), f"colour differs in frame {k}"
         assert np.array_equal(gdepth.view(np.uint32), odepth.view(np.uint32)), f"depth differs in frame {k}"
         assert (odepth > 0).sum() > 0
+
+
+def test_engines_are_the_ones_asked_for():
+    """The default State compiles the tape; the interpreter runs only when asked."""
+    tape = tape_of("spheres")
+    with T.State(tape, localsubdiv=4, voxel_capacity=1 << 16) as st:
+        info = st.tape_info()
+        assert info["engine"] == "compiled" and info["n"] == len(tape)
+    with T.State(tape, localsubdiv=4, voxel_capacity=1 << 16, engine="interpreter") as st:
+        assert st.tape_info()["engine"] == "interpreter"
