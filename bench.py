@@ -68,6 +68,60 @@ def cpu_baseline(tape, w, budget_s: float):
                       f"{dt:.1f} s"}
 
 
+def _rocpd_kernels(db):
+    """(name, dispatches, total ns) per kernel of a rocprofv3 --kernel-trace database."""
+    import sqlite3
+
+    c = sqlite3.connect(db)
+    agg = {}
+    for name, d in c.execute("select name, duration from kernels"):   # ns per dispatch
+        a = agg.setdefault(name, [0, 0.0])
+        a[0] += 1
+        a[1] += float(d)
+    return sorted(((n, k, t) for n, (k, t) in agg.items()), key=lambda r: -r[2])
+
+
+def _rocpd_counter(db):
+    """kernel short name -> (dispatches, summed counter value in KiB) of a --pmc run."""
+    import sqlite3
+
+    c = sqlite3.connect(db)
+    out = {}
+    for name, v in c.execute("select name, counter_value from pmc_events"):
+        k = name.split("(")[0].replace("void ", "")
+        a = out.setdefault(k, [0, 0.0])
+        a[0] += 1
+        a[1] += v
+    return out
+
+
+def summarize_profile(args):
+    """Text summary of a rocprofv3 --kernel-trace --stats run and, when given, of the
+    FETCH_SIZE / WRITE_SIZE passes of the same command and of the copy calibration."""
+    lines = ["kernel                                                   dispatches   total_ms    avg_us   share"]
+    top = _rocpd_kernels(args.summarize)
+    total = sum(t for _, _, t in top) or 1.0
+    for n, k, t in top[:14]:
+        lines.append(f"{n.split('(')[0][:56]:56s} {k:10d} {t / 1e6:10.3f} {t / k / 1e3:9.2f} {100 * t / total:6.1f}%")
+    if args.pmc_fetch and args.pmc_write and args.copy_fetch and args.copy_write:
+        mib = args.copy_mib
+        cf = _rocpd_counter(args.copy_fetch)["__amd_rocclr_copyBuffer"]
+        cw = _rocpd_counter(args.copy_write)["__amd_rocclr_copyBuffer"]
+        f_scale = (mib * 1024.0) / (cf[1] / cf[0])
+        w_scale = (mib * 1024.0) / (cw[1] / cw[0])
+        lines.append("")
+        lines.append(f"copy calibration ({mib} MiB device-to-device): FETCH_SIZE reads {cf[1] / cf[0]:.0f} KiB "
+                     f"-> scale {f_scale:.3f}; WRITE_SIZE reads {cw[1] / cw[0]:.0f} KiB -> scale {w_scale:.3f}")
+        fe, wr = _rocpd_counter(args.pmc_fetch), _rocpd_counter(args.pmc_write)
+        lines.append("kernel                                 dispatches  fetch_KiB/disp  write_KiB/disp  hbm_bytes/disp")
+        for k in sorted(fe, key=lambda k: -fe[k][1])[:8]:
+            n, fkib = fe[k]
+            wkib = wr.get(k, [n, 0.0])[1] / max(wr.get(k, [n, 0.0])[0], 1)
+            hbm = (fkib / n * f_scale + wkib * w_scale) * 1024.0
+            lines.append(f"{k[:38]:38s} {n:10d} {fkib / n * f_scale:15.1f} {wkib * w_scale:15.1f} {hbm:15.0f}")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +135,31 @@ def main():
     ap.add_argument("--prune", default="on", choices=["on", "off", "exact"],
                     help="pruned per-group programs: on (default), off (full tape per cell), exact "
                          "(only for tapes whose pruned results are provably identical)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="instead of the benchmark: copy a buffer of this many MiB device-to-device 5 times "
+                         "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--summarize", metavar="KERNEL_TRACE_DB",
+                    help="instead of the benchmark: summarise a rocprofv3 --kernel-trace --stats database")
+    ap.add_argument("--pmc-fetch", metavar="DB")
+    ap.add_argument("--pmc-write", metavar="DB")
+    ap.add_argument("--copy-fetch", metavar="DB")
+    ap.add_argument("--copy-write", metavar="DB")
+    ap.add_argument("--copy-mib", type=int, default=256)
     args = ap.parse_args()
+    if args.summarize:
+        summarize_profile(args)
+        return
+    if args.calibrate_copy_mib:
+        import torch
+
+        n = args.calibrate_copy_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs the "
               f"set-inversion path on a single MI355X (it would shard by octree subtree)", file=sys.stderr)

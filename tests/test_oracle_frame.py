@@ -1,0 +1,128 @@
+"""Pins of the oracle's whole-frame restatement (oracle/frame.c: compute.wgsl `main` as a FIFO).
+
+* First frame (every cell new, degenerate t-window): the leaf set equals a queue-free
+  recursive subdivision written here from the rules of compute.wgsl:3521, 3582-3631, 4011-4034.
+* FP64 dense sampling: no cell classified empty (f = 1) or full (f = -1) contains a sign change
+  of the float64 field, beyond a few binary32 ulps of zero (the intervals are not rounded outward);
+  point-sampled leaves are classified from their centre and are not covered.
+* Multi-frame structure: children come in complete blocks of 8, leaves sit at depth <= L.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from temporalsetinversion_amd import scenes
+
+import llinterp
+
+BOX = (-5.0, 5.0)
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def naive_first_frame(tape, L, t, point_sample=True):
+    """Recursive restatement: returns {(depth, i, j, k): (f, has_children)} for every evaluated cell."""
+    eps = f32(10.0) * f32(2.0 ** -L)                                   # compute.wgsl:3521
+    t = float(f32(t))
+    out = {}
+
+    def visit(depth, ijk, lo, hi):
+        d = [f32((f32(hi[a]) - f32(lo[a])) / f32(2.0)) for a in range(3)]     # 3582
+        width = max(d)
+        for o in range(8):
+            b = (o & 1, (o >> 1) & 1, o >> 2)                                  # x + 2y + 4z (3576)
+            clo = [f32(lo[a] + d[a] * f32(b[a])) for a in range(3)]
+            chi = [f32(lo[a] + d[a] * f32(1 + b[a])) for a in range(3)]
+            args = [(float(clo[a]), float(chi[a])) for a in range(3)]
+            leaf_w = width
+            if point_sample and width <= eps:                                  # 3616-3621
+                args = [(float(m), float(m)) for m in (f32((clo[a] + chi[a]) / f32(2.0)) for a in range(3))]
+                leaf_w = 0.0
+            (flo, fhi), *_ = oracle.eval_tape(tape, args + [(t, t)], False)
+            key = (depth + 1, 2 * ijk[0] + b[0], 2 * ijk[1] + b[1], 2 * ijk[2] + b[2])
+            if flo <= 0.0 <= fhi:
+                f = 0.0
+                if leaf_w > eps:                                               # 4015-4026 (new cell)
+                    out[key] = (f, True)
+                    visit(depth + 1, key[1:], clo, chi)
+                    continue
+            else:
+                f = -1.0 if fhi <= 0.0 else 1.0
+            out[key] = (f, False)
+
+    visit(0, (0, 0, 0), [f32(BOX[0])] * 3, [f32(BOX[1])] * 3)
+    return out
+
+
+@pytest.mark.parametrize("scene,L", [("sphere", 4), ("spheres_linear", 4), ("spheres", 3), ("capsules", 3)])
+def test_first_frame_equals_recursive_subdivision(scene, L):
+    text = scenes.sphere_unit(1.0) if scene == "sphere" else scenes.SCENES[scene]()
+    tape = oracle.parse_ll(text)
+    fr = oracle.Frame(tape, localsubdiv=L, voxel_capacity=1 << 20)
+    fr.set_inversion(0.25)
+    nodes = fr.export()
+    fr.close()
+    got = {(int(n["depth"]), int(n["i"]), int(n["j"]), int(n["k"])): (float(n["f"]), bool(n["has_children"]))
+           for n in nodes if n["depth"] > 0}
+    want = naive_first_frame(tape, L, 0.25)
+    assert got == want
+    first = nodes[nodes["depth"] > 0]
+    assert np.all(first["t"] == np.float32(0.25)) and np.all(first["time_err"] == 0.0)
+    assert np.all(first["dx"] == 0.0) and np.all(first["fast"] == 0.0)     # no reverse pass on frame 0
+
+
+def cell_box(n, L0=BOX):
+    size = (L0[1] - L0[0]) / (1 << int(n["depth"]))
+    lo = [L0[0] + size * int(n[a]) for a in ("i", "j", "k")]
+    return lo, [x + size for x in lo]
+
+
+@pytest.mark.parametrize("scene,L,times", [("sphere", 4, [0.0]), ("spheres_linear", 4, [0.0, 1 / 60, 0.5]),
+                                           ("capsules", 4, [0.0, 1 / 60])])
+def test_no_sign_change_inside_empty_or_full_cells(scene, L, times):
+    """FP64 dense sampling (5^3 points per cell, at the cell's own evaluation time)."""
+    text = scenes.sphere_unit(1.0) if scene == "sphere" else scenes.SCENES[scene]()
+    tape = oracle.parse_ll(text)
+    prog = llinterp.Program(text)
+    fr = oracle.Frame(tape, localsubdiv=L, voxel_capacity=1 << 20)
+    for t in times:
+        fr.set_inversion(float(np.float32(t)))
+        fr.enqueue_root()
+    nodes = fr.export()
+    fr.close()
+    # cells at depth L are point-sampled at their centre (compute.wgsl:3616-3621): excluded
+    decided = nodes[(nodes["depth"] > 0) & (nodes["depth"] < L) & (nodes["f"] != 0.0)]
+    assert len(decided) > 0
+    grid = np.linspace(0.0, 1.0, 5)
+    tol = 1e-5
+    for n in decided:
+        lo, hi = cell_box(n)
+        for u in grid:
+            for v in grid:
+                for w in grid:
+                    p = [lo[0] + u * (hi[0] - lo[0]), lo[1] + v * (hi[1] - lo[1]), lo[2] + w * (hi[2] - lo[2])]
+                    val, _ = prog.eval(*p, float(n["t"]))
+                    if n["f"] > 0:
+                        assert val > -tol, (scene, tuple(n), p, val)
+                    else:
+                        assert val < tol, (scene, tuple(n), p, val)
+
+
+def test_tree_structure_over_frames():
+    tape = oracle.parse_ll(scenes.SCENES["spheres_linear"]())
+    L = 5
+    fr = oracle.Frame(tape, localsubdiv=L, voxel_capacity=1 << 20)
+    for k in range(6):
+        fr.set_inversion(float(np.float32(k / 30.0)))
+        fr.enqueue_root()
+        nodes = fr.export()
+        keys = {(int(n["depth"]), int(n["i"]), int(n["j"]), int(n["k"])) for n in nodes}
+        assert len(keys) == len(nodes)
+        assert nodes["depth"].max() <= L
+        for n in nodes[nodes["has_children"] == 1]:
+            d, i, j, kk = int(n["depth"]), int(n["i"]), int(n["j"]), int(n["k"])
+            for o in range(8):
+                assert (d + 1, 2 * i + (o & 1), 2 * j + ((o >> 1) & 1), 2 * kk + (o >> 2)) in keys
+    fr.close()
