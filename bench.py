@@ -44,82 +44,64 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(tape, w, budget_s: float):
-    """Oracle (single-threaded C restatement) on a bounded sample: the same tape and
-    parameters at a reduced depth, frames t_k = k/60 until the time budget is spent."""
+def profiled_traffic(workload: str, engine: str):
+    """HBM bytes per eval launch from the committed rocprofv3 summary of this workload, if any
+    (FETCH_SIZE/WRITE_SIZE passes scaled by the copy calibration; see profiles/)."""
+    path = os.path.join(ROOT, "profiles", f"tape_eval_{workload.split('_')[0]}_{workload.split('_', 1)[1]}.txt")
+    if engine != "compiled" or not os.path.exists(path):
+        return None, None
+    rows = {}
+    in_traffic = False
+    for line in open(path):
+        parts = line.split()
+        in_traffic = in_traffic or "hbm_bytes/disp" in line
+        if in_traffic and len(parts) == 5 and parts[0].startswith("tsi_tape_eval_"):
+            rows[parts[0]] = (int(parts[1]), float(parts[4]))
+    if not rows:
+        return None, None
+    n = sum(k for k, _ in rows.values())
+    return sum(k * b for k, b in rows.values()) / n, os.path.relpath(path, ROOT)
+
+
+def _cpu_worker(job):
+    """One oracle frame sequence (frames t_k = (k + phase)/60) until the time budget is spent."""
     import oracle
 
-    L = max(3, w["localsubdiv"] - 3)
+    ll, w, L, phase, budget_s = job
+    tape = oracle.parse_ll(ll)
     fr = oracle.Frame(tape, localsubdiv=L, swept_volume=w["swept_volume"],
                       lipschitz_corrected=w["lipschitz_corrected"], point_sample=w["point_sample"],
                       voxel_capacity=1 << 22)
-    cells = 0
-    frames = 0
+    cells = frames = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and frames < 120:
-        st = fr.set_inversion(float(np.float32(frames / 60.0)))
+    while time.perf_counter() - t0 < budget_s and frames < 240:
+        st = fr.set_inversion(float(np.float32((frames + phase) / 60.0)))
         cells += st.cell_evals
         frames += 1
         fr.enqueue_root()
     dt = time.perf_counter() - t0
     fr.close()
-    return {"value": cells / dt, "unit": "cell-evals/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, same tape at localsubdiv {L}, {frames} frames t=k/60, {cells} cell-evals, "
-                      f"{dt:.1f} s"}
+    return cells, frames, dt
 
 
-def _rocpd_kernels(db):
-    """(name, dispatches, total ns) per kernel of a rocprofv3 --kernel-trace database."""
-    import sqlite3
+def cpu_baseline(ll, w, budget_s: float):
+    """The oracle (C restatement of compute.wgsl `main`) on the host cores: one independent frame
+    sequence per worker process (phases 7 frames apart), same tape and parameters at localsubdiv
+    L-3, each until the time budget is spent.  Runs before the GPU is initialised."""
+    import multiprocessing as mp
 
-    c = sqlite3.connect(db)
-    agg = {}
-    for name, d in c.execute("select name, duration from kernels"):   # ns per dispatch
-        a = agg.setdefault(name, [0, 0.0])
-        a[0] += 1
-        a[1] += float(d)
-    return sorted(((n, k, t) for n, (k, t) in agg.items()), key=lambda r: -r[2])
-
-
-def _rocpd_counter(db):
-    """kernel short name -> (dispatches, summed counter value in KiB) of a --pmc run."""
-    import sqlite3
-
-    c = sqlite3.connect(db)
-    out = {}
-    for name, v in c.execute("select name, counter_value from pmc_events"):
-        k = name.split("(")[0].replace("void ", "")
-        a = out.setdefault(k, [0, 0.0])
-        a[0] += 1
-        a[1] += v
-    return out
-
-
-def summarize_profile(args):
-    """Text summary of a rocprofv3 --kernel-trace --stats run and, when given, of the
-    FETCH_SIZE / WRITE_SIZE passes of the same command and of the copy calibration."""
-    lines = ["kernel                                                   dispatches   total_ms    avg_us   share"]
-    top = _rocpd_kernels(args.summarize)
-    total = sum(t for _, _, t in top) or 1.0
-    for n, k, t in top[:14]:
-        lines.append(f"{n.split('(')[0][:56]:56s} {k:10d} {t / 1e6:10.3f} {t / k / 1e3:9.2f} {100 * t / total:6.1f}%")
-    if args.pmc_fetch and args.pmc_write and args.copy_fetch and args.copy_write:
-        mib = args.copy_mib
-        cf = _rocpd_counter(args.copy_fetch)["__amd_rocclr_copyBuffer"]
-        cw = _rocpd_counter(args.copy_write)["__amd_rocclr_copyBuffer"]
-        f_scale = (mib * 1024.0) / (cf[1] / cf[0])
-        w_scale = (mib * 1024.0) / (cw[1] / cw[0])
-        lines.append("")
-        lines.append(f"copy calibration ({mib} MiB device-to-device): FETCH_SIZE reads {cf[1] / cf[0]:.0f} KiB "
-                     f"-> scale {f_scale:.3f}; WRITE_SIZE reads {cw[1] / cw[0]:.0f} KiB -> scale {w_scale:.3f}")
-        fe, wr = _rocpd_counter(args.pmc_fetch), _rocpd_counter(args.pmc_write)
-        lines.append("kernel                                 dispatches  fetch_KiB/disp  write_KiB/disp  hbm_bytes/disp")
-        for k in sorted(fe, key=lambda k: -fe[k][1])[:8]:
-            n, fkib = fe[k]
-            wkib = wr.get(k, [n, 0.0])[1] / max(wr.get(k, [n, 0.0])[0], 1)
-            hbm = (fkib / n * f_scale + wkib * w_scale) * 1024.0
-            lines.append(f"{k[:38]:38s} {n:10d} {fkib / n * f_scale:15.1f} {wkib * w_scale:15.1f} {hbm:15.0f}")
-    print("\n".join(lines))
+    L = max(3, w["localsubdiv"] - 3)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_worker, [(ll, w, L, 7 * i, budget_s) for i in range(workers)])
+    wall = time.perf_counter() - t0
+    cells = sum(r[0] for r in res)
+    frames = sum(r[1] for r in res)
+    return {"value": cells / wall, "unit": "cell-evals/s", "cores": workers, "kind": "port",
+            "sample": f"oracle (C restatement) on {workers} host cores, one frame sequence per core, same tape "
+                      f"and parameters at localsubdiv {L}, {frames} frames, {cells} cell-evals in {wall:.1f} s"}
 
 
 def main():
@@ -128,10 +110,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="capsules64_L9_1024", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline worker")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", default="compiled", choices=["compiled", "interpreter"],
                     help="tape evaluation: compiled straight-line kernels (default) or the interpreter")
+    ap.add_argument("--sync-trace", action="store_true",
+                    help="trace each frame synchronously instead of overlapping it with the next set inversion")
     ap.add_argument("--prune", default="on", choices=["on", "off", "exact"],
                     help="pruned per-group programs: on (default), off (full tape per cell), exact "
                          "(only for tapes whose pruned results are provably identical)")
@@ -165,11 +149,14 @@ def main():
               f"set-inversion path on a single MI355X (it would shard by octree subtree)", file=sys.stderr)
         sys.exit(2)
 
-    import temporalsetinversion_amd as T
     from temporalsetinversion_amd import scenes
 
     w = WORKLOADS[args.workload]
     ll = scenes.SCENES[w["scene"]](w["seed"])
+    cpu = None if args.no_cpu else cpu_baseline(ll, w, args.cpu_budget)
+
+    import temporalsetinversion_amd as T
+
     tape = T.tape_from_ll(ll)
     st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
                  point_sample=w["point_sample"], swept_volume=w["swept_volume"],
@@ -180,15 +167,21 @@ def main():
     frame = 0
 
     def step():
+        # the trace of frame k runs on a second stream while frame k+1's set inversion runs
         nonlocal frame
         st.update(float(np.float32(frame / 60.0)))
         r = st.set_inversion()
-        st.render(pixels=False)
+        if args.sync_trace:
+            st.render(pixels=False)
+        else:
+            st.render_async()
         frame += 1
         return r
 
     for _ in range(args.warmup):
         step()
+    st.trace_wait(pixels=False)
+    ms0, visits0, n0 = st.trace_totals()
     tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0, pruned=0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -202,11 +195,21 @@ def main():
         tot["eval_ms"] += s["eval_ms"]
         tot["launches"] += s["eval_launches"]
         tot["pruned"] += s["pruned_instr"]
+    st.trace_wait(pixels=False)   # the last frame's trace is part of the timed work
     wall = time.perf_counter() - t0
+    ms1, visits1, n1 = st.trace_totals()
+    assert n1 - n0 == args.steps
+    trace = dict(ms=ms1 - ms0, visits=visits1 - visits0)
     st.close()
 
     ops = tot["fwd_ops"] + tot["rev_ops"]
     achieved = ops / (tot["eval_ms"] * 1e-3) / 1e12 if tot["eval_ms"] > 0 else 0.0
+    launches = max(tot["launches"], 1)
+    traffic, traffic_src = profiled_traffic(args.workload, info["engine"])
+    # trace: SURVEY-style algorithmic bytes, 40 B per node visit + 20 B per pixel (colour + depth)
+    px = w["width"] * w["height"]
+    trace_bytes = trace["visits"] * 40 + args.steps * px * 20
+    trace_gbs = trace_bytes / (trace["ms"] * 1e-3) / 1e9 if trace["ms"] > 0 else 0.0
     out = {
         "metric": "interval cell-evals/s",
         "value": tot["cells"] / wall,
@@ -224,6 +227,7 @@ def main():
                    "frame": f"{w['width']}x{w['height']}", "delta": 0.01,
                    "lipschitz_corrected": w["lipschitz_corrected"], "swept_volume": w["swept_volume"],
                    "point_sample": w["point_sample"], "times": "t_k = k/60", "engine": info["engine"],
+                   "trace": "synchronous" if args.sync_trace else "overlapped with the next set inversion",
                    "prune": args.prune if info["engine"] == "interpreter" else "n/a"},
         "tape_compile_s": info["compile_ms"] / 1e3, "tape_compile_cached": bool(info["cache_hit"]),
         "records_run_per_cell": (len(tape) - 4) - tot["pruned"] / max(tot["cells"], 1),
@@ -231,17 +235,24 @@ def main():
         "cell_evals_per_frame": tot["cells"] / args.steps,
         "rev_evals_per_frame": tot["rev"] / args.steps,
         "frames_per_s": args.steps / wall,
-        "roofline": {"bound": "valu", "kernel": "k_eval (tape evaluation, forward + reverse)",
+        "roofline": {"bound": "valu",
+                     "kernel": ("tsi_tape_eval_fwd/_rev" if info["engine"] == "compiled" else "k_eval<false>/<true>")
+                               + " (one forward-only and one reverse level kernel per octree level)",
                      "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "T f32-ops/s",
-                     "frac": achieved / PEAK_VALU_TOPS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic, "traffic_unit": "HBM bytes per launch",
+                     "traffic_source": traffic_src,
+                     "algorithmic_ops_per_launch": ops / launches,
+                     "avg_launch_us": tot["eval_ms"] * 1e3 / launches,
                      "peak_unpacked": PEAK_VALU_UNPACKED_TOPS,
                      "eval_ms_per_frame": tot["eval_ms"] / args.steps,
                      "ops_per_frame": ops / args.steps},
+        "roofline_trace": {"bound": "hbm", "kernel": "k_trace", "achieved": trace_gbs, "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": trace_gbs / PEAK_HBM_GBS,
+                           "trace_ms_per_frame": trace["ms"] / args.steps,
+                           "node_visits_per_frame": trace["visits"] / args.steps},
     }
-    if not args.no_cpu:
-        import oracle
-
-        out["cpu_baseline"] = cpu_baseline(oracle.parse_ll(ll), w, args.cpu_budget)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

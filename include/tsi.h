@@ -140,6 +140,16 @@ int tsi_set_inversion(tsi_ctx *ctx, double *gpu_ms, tsi_stats *stats);
  * and depth_out (W*H floats, -1 = miss); either may be NULL. */
 int tsi_trace(tsi_ctx *ctx, float *rgba_out, float *depth_out);
 
+/* The same trace without waiting (no counterpart in the reference, whose render call is
+ * synchronous): the nodes in use are copied to a device-side snapshot in stream order after
+ * the last set inversion, and a second stream traces the snapshot while the caller goes on
+ * with the next frame's tsi_set_inversion.  tsi_trace_wait waits for it and copies the frame
+ * out (rgba / depth may be NULL).  A new tsi_trace_async first waits for the previous one. */
+int tsi_trace_async(tsi_ctx *ctx);
+int tsi_trace_wait(tsi_ctx *ctx, float *rgba_out, float *depth_out);
+/* Device time and node visits summed over the traces completed since the context was made. */
+int tsi_trace_totals(tsi_ctx *ctx, double *trace_ms, uint64_t *node_visits, uint64_t *traces);
+
 /* Device pointers of the frame written by the last tsi_trace (valid until the next
  * tsi_trace or tsi_destroy), for callers that keep the frame on the GPU. */
 int tsi_frame_device(tsi_ctx *ctx, const float **rgba_dev, const float **depth_dev);
@@ -155,6 +165,10 @@ int tsi_read_nodes(tsi_ctx *ctx, tsi_node *out, uint64_t cap, uint64_t *count);
 
 /* The uploaded tape as 32-byte records (src/main.rs:822-876 tape dump). */
 int tsi_read_tape(tsi_ctx *ctx, tsi_gpu_instruction *out, uint32_t cap, uint32_t *n);
+
+/* Device time of the last tsi_trace dispatch and the octree nodes it visited (child-state
+ * loads of the traversal; no counterpart in the reference, which has no timer on rt_main). */
+int tsi_last_trace_info(tsi_ctx *ctx, double *trace_ms, uint64_t *node_visits);
 
 /* How the uploaded tape is evaluated (no counterpart in the reference, which interprets). */
 typedef struct {

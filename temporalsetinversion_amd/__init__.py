@@ -210,6 +210,34 @@ class State:
                               depth.ctypes.data if pixels else None), "tsi_trace")
         return rgba, depth
 
+    def last_trace(self) -> tuple[float, int]:
+        """(device ms, octree node visits) of the last trace."""
+        ms, visits = C.c_double(0.0), C.c_uint64(0)
+        check(lib().tsi_last_trace_info(self._ctx, C.byref(ms), C.byref(visits)), "tsi_last_trace_info")
+        return ms.value, visits.value
+
+    def trace_async(self):
+        """Starts tracing the current octree (a device snapshot) on a second stream."""
+        check(lib().tsi_trace_async(self._ctx), "tsi_trace_async")
+
+    def trace_wait(self, pixels: bool = True):
+        rgba = np.zeros((self.height, self.width, 4), np.float32) if pixels else None
+        depth = np.zeros((self.height, self.width), np.float32) if pixels else None
+        check(lib().tsi_trace_wait(self._ctx, rgba.ctypes.data if pixels else None,
+                                   depth.ctypes.data if pixels else None), "tsi_trace_wait")
+        return rgba, depth
+
+    def render_async(self):
+        """rt_main of the current octree without waiting for it, then buf_main."""
+        self.trace_async()
+        self.enqueue_root()
+
+    def trace_totals(self) -> tuple[float, int, int]:
+        """(device ms, node visits, traces) summed over the completed traces."""
+        ms, visits, n = C.c_double(0.0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().tsi_trace_totals(self._ctx, C.byref(ms), C.byref(visits), C.byref(n)), "tsi_trace_totals")
+        return ms.value, visits.value, n.value
+
     def enqueue_root(self):
         check(lib().tsi_enqueue_root(self._ctx), "tsi_enqueue_root")
 

@@ -85,6 +85,10 @@ SIGNATURES = {
     "tsi_set_uniforms": (C.c_int, [C.c_void_p, C.POINTER(Uniforms)]),
     "tsi_set_inversion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(Stats)]),
     "tsi_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tsi_trace_async": (C.c_int, [C.c_void_p]),
+    "tsi_trace_wait": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tsi_trace_totals": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]),
     "tsi_frame_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "tsi_enqueue_root": (C.c_int, [C.c_void_p]),
     "tsi_read_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -93,6 +97,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_uint32)]),
     "tsi_camera_zdist": (C.c_float, [C.c_uint32]),
     "tsi_tape_info_get": (C.c_int, [C.c_void_p, C.POINTER(TapeInfo)]),
+    "tsi_last_trace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "tsi_precompile_tape": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.POINTER(TapeInfo)]),
     "tsi_strerror": (C.c_char_p, [C.c_int]),
 }

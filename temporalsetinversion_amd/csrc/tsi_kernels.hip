@@ -366,8 +366,11 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
 
 __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     __shared__ uint32_t stack_lds[16 * 256];
-    const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
-    const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+    // a 16x16 block of pixels; each wavefront takes one 8x8 quadrant (square tiles keep the
+    // rays of a wavefront close together, so their traversals diverge less)
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    const uint32_t x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
+    const uint32_t y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
     uint32_t visits = 0;
     if (x < t.width && y < t.height) {
         // ray_direction (compute.wgsl:4590-4594) and view * vec4(d, 0) (4630)

@@ -30,7 +30,7 @@ struct tsi_ctx {
     int device = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t nodes = 0;
     uint32_t nblocks = 0, list_cap = 0, dlist_cap = 0;
     float4 *st = nullptr, *blo = nullptr, *bhi = nullptr, *grad = nullptr;
@@ -63,11 +63,24 @@ struct tsi_ctx {
     tsi_uniforms u{};
     uint32_t grid_fwd = 0, grid_rev = 0, grid_misc = 0;
     uint64_t last_visits = 0;
+    double last_trace_ms = 0.0;
+    // asynchronous trace: a second stream traces a device-side snapshot of the octree while the
+    // next frame's set inversion runs on the main stream
+    hipStream_t tstream = nullptr;
+    hipEvent_t ev_snap = nullptr, ev_tdone = nullptr;
+    float4 *snap_st = nullptr, *snap_blo = nullptr, *snap_bhi = nullptr;
+    uint64_t snap_cap = 0;
+    uint64_t used_nodes = 8;            // nodes in use after the last set inversion
+    bool trace_pending = false;
+    double trace_ms_total = 0.0;
+    uint64_t trace_visits_total = 0, trace_count = 0;
 };
 
 namespace {
 
 #define HIP_OK(x) do { if ((x) != hipSuccess) return TSI_E_HIP; } while (0)
+
+int harvest_trace(tsi_ctx *c);
 
 template <class T>
 int dmalloc(T **p, size_t count) {
@@ -221,8 +234,13 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
 }
 
 void release(tsi_ctx *c) {
+    if (c->tstream) hipStreamSynchronize(c->tstream);
     release_jit(c);
     release_prune(c);
+    dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
+    if (c->ev_snap) { hipEventDestroy(c->ev_snap); c->ev_snap = nullptr; }
+    if (c->ev_tdone) { hipEventDestroy(c->ev_tdone); c->ev_tdone = nullptr; }
+    if (c->tstream) { hipStreamDestroy(c->tstream); c->tstream = nullptr; }
     dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
@@ -288,7 +306,13 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = TSI_E_HIP; break; }
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&c->ev_tdone) != hipSuccess) {
+            rc = TSI_E_HIP;
+            break;
+        }
         bool evok = true;
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         if (!evok) { rc = TSI_E_HIP; break; }
@@ -337,6 +361,7 @@ void tsi_destroy(tsi_ctx *c) {
 
 int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     if (!c || !tape) return TSI_E_ARG;
+    if (harvest_trace(c)) return TSI_E_HIP;
     if (n > 4096) return TSI_E_TAPE_TOO_LONG;
     std::vector<FwdIns> fwd;
     std::vector<RevIns> rev;
@@ -391,6 +416,7 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
 
 int tsi_reset(tsi_ctx *c, const float root_box[6]) {
     if (!c || !root_box) return TSI_E_ARG;
+    if (harvest_trace(c)) return TSI_E_HIP;
     KArgs a = make_args(c);
     if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -441,17 +467,41 @@ int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         stats->eval_launches = uint64_t(2 * c->p.localsubdiv);
         stats->prune_launches = c->prune ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
     }
+    c->used_nodes = (uint64_t(h.bump) + 1u) * 8u;
+    if (c->used_nodes > c->nodes) c->used_nodes = c->nodes;
     if (h.frame_error & kErrPool) return TSI_E_POOL_EXHAUSTED;
     if (h.frame_error & kErrQueue) return TSI_E_QUEUE_FULL;
     return TSI_OK;
 }
 
-int tsi_trace(tsi_ctx *c, float *rgba_out, float *depth_out) {
+namespace {
+
+// Waits for the pending asynchronous trace and records its device time and node visits.
+int harvest_trace(tsi_ctx *c) {
+    if (!c->trace_pending) return TSI_OK;
+    c->trace_pending = false;
+    HIP_OK(hipEventSynchronize(c->ev_tdone));
+    unsigned long long v = 0;
+    HIP_OK(hipMemcpy(&v, c->visits, sizeof v, hipMemcpyDeviceToHost));
+    float tms = 0.0f;
+    hipEventElapsedTime(&tms, c->ev[4], c->ev_tdone);
+    c->last_visits = v;
+    c->last_trace_ms = tms;
+    c->trace_ms_total += tms;
+    c->trace_visits_total += v;
+    c->trace_count += 1;
+    return TSI_OK;
+}
+
+}  // namespace
+
+int tsi_trace_async(tsi_ctx *c) {
     if (!c) return TSI_E_ARG;
     const uint32_t W = c->u.resolution[0], H = c->u.resolution[1];
     if (W == 0 || H == 0) return TSI_E_ARG;
-    const size_t px = size_t(W) * H;
     int rc;
+    if ((rc = harvest_trace(c))) return rc;   // the snapshot and frame buffers are free again
+    const size_t px = size_t(W) * H;
     if (px > c->frame_px) {
         dfree(c->rgba);
         dfree(c->depth);
@@ -459,6 +509,22 @@ int tsi_trace(tsi_ctx *c, float *rgba_out, float *depth_out) {
         if ((rc = dmalloc(&c->rgba, px)) || (rc = dmalloc(&c->depth, px))) return rc;
         c->frame_px = px;
     }
+    const uint64_t used = c->used_nodes;
+    if (used > c->snap_cap) {
+        uint64_t cap = used * 2u < c->nodes ? used * 2u : c->nodes;
+        if (cap < used) cap = used;
+        dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
+        c->snap_cap = 0;
+        if ((rc = dmalloc(&c->snap_st, cap)) || (rc = dmalloc(&c->snap_blo, cap)) || (rc = dmalloc(&c->snap_bhi, cap)))
+            return rc;
+        c->snap_cap = cap;
+    }
+    // snapshot of the nodes in use, in stream order after the last set inversion
+    HIP_OK(hipMemcpyAsync(c->snap_st, c->st, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->snap_blo, c->blo, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->snap_bhi, c->bhi, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_OK(hipEventRecord(c->ev_snap, c->stream));
+    HIP_OK(hipStreamWaitEvent(c->tstream, c->ev_snap, 0));
     TraceArgs t{};
     std::memcpy(t.view, c->u.view, sizeof t.view);
     t.width = W;
@@ -468,14 +534,40 @@ int tsi_trace(tsi_ctx *c, float *rgba_out, float *depth_out) {
     t.rgba = c->rgba;
     t.depth = c->depth;
     t.visits = c->visits;
-    HIP_OK(hipMemsetAsync(c->visits, 0, sizeof(unsigned long long), c->stream));
-    if (launch_trace(make_args(c), t, c->stream) != 0) return TSI_E_HIP;
-    if (rgba_out) HIP_OK(hipMemcpyAsync(rgba_out, c->rgba, px * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    if (depth_out) HIP_OK(hipMemcpyAsync(depth_out, c->depth, px * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    unsigned long long v = 0;
-    HIP_OK(hipMemcpyAsync(&v, c->visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    c->last_visits = v;
+    KArgs a = make_args(c);
+    a.st = c->snap_st;
+    a.blo = c->snap_blo;
+    a.bhi = c->snap_bhi;
+    HIP_OK(hipMemsetAsync(c->visits, 0, sizeof(unsigned long long), c->tstream));
+    HIP_OK(hipEventRecord(c->ev[4], c->tstream));
+    if (launch_trace(a, t, c->tstream) != 0) return TSI_E_HIP;
+    HIP_OK(hipEventRecord(c->ev_tdone, c->tstream));
+    c->trace_pending = true;
+    return TSI_OK;
+}
+
+int tsi_trace_wait(tsi_ctx *c, float *rgba_out, float *depth_out) {
+    if (!c) return TSI_E_ARG;
+    int rc;
+    if ((rc = harvest_trace(c))) return rc;
+    const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
+    if ((rgba_out || depth_out) && px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
+    if (rgba_out) HIP_OK(hipMemcpy(rgba_out, c->rgba, px * sizeof(float4), hipMemcpyDeviceToHost));
+    if (depth_out) HIP_OK(hipMemcpy(depth_out, c->depth, px * sizeof(float), hipMemcpyDeviceToHost));
+    return TSI_OK;
+}
+
+int tsi_trace(tsi_ctx *c, float *rgba_out, float *depth_out) {
+    int rc = tsi_trace_async(c);
+    if (rc) return rc;
+    return tsi_trace_wait(c, rgba_out, depth_out);
+}
+
+int tsi_trace_totals(tsi_ctx *c, double *trace_ms, uint64_t *node_visits, uint64_t *traces) {
+    if (!c) return TSI_E_ARG;
+    if (trace_ms) *trace_ms = c->trace_ms_total;
+    if (node_visits) *node_visits = c->trace_visits_total;
+    if (traces) *traces = c->trace_count;
     return TSI_OK;
 }
 
@@ -536,6 +628,13 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_in
     }
     if (out) *out = info;
     return rc;
+}
+
+int tsi_last_trace_info(tsi_ctx *c, double *trace_ms, uint64_t *node_visits) {
+    if (!c) return TSI_E_ARG;
+    if (trace_ms) *trace_ms = c->last_trace_ms;
+    if (node_visits) *node_visits = c->last_visits;
+    return TSI_OK;
 }
 
 int tsi_tape_info_get(tsi_ctx *c, tsi_tape_info *out) {
