@@ -166,6 +166,13 @@ int tsi_read_nodes(tsi_ctx *ctx, tsi_node *out, uint64_t cap, uint64_t *count);
 /* The uploaded tape as 32-byte records (src/main.rs:822-876 tape dump). */
 int tsi_read_tape(tsi_ctx *ctx, tsi_gpu_instruction *out, uint32_t cap, uint32_t *n);
 
+/* The frame dump of src/main.rs:693-715: the last traced frame converted to RGBA8 (clamp to
+ * [0,1], x255, round half away from zero, as image 0.24 `into_rgba8`; row 0 = top), copied to
+ * rgba8_out (W*H*4 bytes), or written as a PNG file.  tsi_write_png writes any RGBA8 image. */
+int tsi_frame_rgba8(tsi_ctx *ctx, uint8_t *rgba8_out);
+int tsi_save_frame_png(tsi_ctx *ctx, const char *path);
+int tsi_write_png(const char *path, const uint8_t *rgba8, uint32_t width, uint32_t height);
+
 /* Device time of the last tsi_trace dispatch and the octree nodes it visited (child-state
  * loads of the traversal; no counterpart in the reference, which has no timer on rt_main). */
 int tsi_last_trace_info(tsi_ctx *ctx, double *trace_ms, uint64_t *node_visits);

@@ -46,6 +46,30 @@ def tape_from_ll(text: str | bytes, time: float = 0.0) -> np.ndarray:
     return np.frombuffer(bytes(out), dtype=INSTR_DTYPE, count=cap).copy()
 
 
+def write_png(path: str, rgba8: np.ndarray):
+    """Writes an RGBA8 image (H x W x 4, row 0 on top) as a PNG."""
+    a = np.ascontiguousarray(rgba8, np.uint8)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected an H x W x 4 uint8 array")
+    check(lib().tsi_write_png(str(path).encode(), a.ctypes.data, a.shape[1], a.shape[0]), "tsi_write_png")
+
+
+def _rust_f64_debug(x: float) -> str:
+    """Rust's `{:?}` of an f64: shortest round-trip digits, '.0' on integral values, exponent
+    without padding ('1e-7', '1e16')."""
+    r = repr(float(x))
+    if "e" in r:
+        m, e = r.split("e")
+        return f"{m}e{int(e)}"
+    return r
+
+
+def timing_line(frame_number: int, rootfinding_ms_accumulated: float, frame_ms: float) -> str:
+    """One line of the reference's timing.txt (main.rs:1071-1073)."""
+    return (f"framenumber: {int(frame_number)}, rootfinding_time_accumulated: "
+            f"{_rust_f64_debug(rootfinding_ms_accumulated)}, actual_frame_time: {_rust_f64_debug(frame_ms)}\n")
+
+
 def precompile(tape) -> dict:
     """Compiles a tape's evaluation kernels into the code-object cache (no GPU needed)."""
     if isinstance(tape, IR):
@@ -237,6 +261,16 @@ class State:
         ms, visits, n = C.c_double(0.0), C.c_uint64(0), C.c_uint64(0)
         check(lib().tsi_trace_totals(self._ctx, C.byref(ms), C.byref(visits), C.byref(n)), "tsi_trace_totals")
         return ms.value, visits.value, n.value
+
+    def frame_rgba8(self) -> np.ndarray:
+        """The last traced frame as RGBA8 (main.rs:709-711 conversion), rows top to bottom."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        check(lib().tsi_frame_rgba8(self._ctx, out.ctypes.data), "tsi_frame_rgba8")
+        return out
+
+    def save_png(self, path: str):
+        """main.rs:693-715 frame dump (images/{frame}.png in the reference)."""
+        check(lib().tsi_save_frame_png(self._ctx, str(path).encode()), "tsi_save_frame_png")
 
     def enqueue_root(self):
         check(lib().tsi_enqueue_root(self._ctx), "tsi_enqueue_root")

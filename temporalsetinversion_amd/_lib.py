@@ -89,6 +89,9 @@ SIGNATURES = {
     "tsi_trace_wait": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tsi_trace_totals": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),
+    "tsi_frame_rgba8": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tsi_save_frame_png": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "tsi_write_png": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "tsi_frame_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "tsi_enqueue_root": (C.c_int, [C.c_void_p]),
     "tsi_read_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),

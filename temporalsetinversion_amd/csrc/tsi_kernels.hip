@@ -396,7 +396,24 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
 }
 
+// RGBA32F -> RGBA8 as the reference's frame dump converts it (src/main.rs:709-711, image 0.24
+// into_rgba8): clamp to [0, 1], scale by 255, round half away from zero.
+__global__ void __launch_bounds__(256) k_rgba8(const float4 *in, uchar4 *out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = in[i];
+    auto q = [](float x) -> unsigned char {
+        return (unsigned char)roundf(wmin(wmax(x, 0.0f), 1.0f) * 255.0f);
+    };
+    out[i] = make_uchar4(q(v.x), q(v.y), q(v.z), q(v.w));
+}
+
 // ---------------------------------------------------------------- launchers
+
+int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_rgba8, dim3((n + 255u) / 256u), dim3(256), 0, s, in, out, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1) {
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a);

@@ -571,6 +571,36 @@ int tsi_trace_totals(tsi_ctx *c, double *trace_ms, uint64_t *node_visits, uint64
     return TSI_OK;
 }
 
+int tsi_frame_rgba8(tsi_ctx *c, uint8_t *rgba8_out) {
+    if (!c || !rgba8_out) return TSI_E_ARG;
+    int rc;
+    if ((rc = harvest_trace(c))) return rc;
+    const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
+    if (px == 0 || px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
+    uchar4 *tmp = nullptr;
+    if ((rc = dmalloc(&tmp, px))) return rc;
+    rc = launch_rgba8(c->rgba, tmp, uint32_t(px), c->stream) == 0 ? TSI_OK : TSI_E_HIP;
+    if (rc == TSI_OK && hipMemcpyAsync(rgba8_out, tmp, px * 4u, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = TSI_E_HIP;
+    if (rc == TSI_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TSI_E_HIP;
+    dfree(tmp);
+    return rc;
+}
+
+int tsi_save_frame_png(tsi_ctx *c, const char *path) {
+    if (!c || !path) return TSI_E_ARG;
+    const uint32_t W = c->u.resolution[0], H = c->u.resolution[1];
+    std::vector<uint8_t> px;
+    try {
+        px.resize(size_t(W) * H * 4u);
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    int rc = tsi_frame_rgba8(c, px.data());
+    if (rc) return rc;
+    return tsi_write_png(path, px.data(), W, H);
+}
+
 int tsi_frame_device(tsi_ctx *c, const float **rgba_dev, const float **depth_dev) {
     if (!c) return TSI_E_ARG;
     if (rgba_dev) *rgba_dev = reinterpret_cast<const float *>(c->rgba);

@@ -134,3 +134,19 @@ def test_overlapped_trace_matches_oracle():
     for k, ((g_rgba, g_depth), (o_rgba, o_depth)) in enumerate(zip(got, want)):
         assert np.array_equal(g_rgba.view(np.uint32), o_rgba.view(np.uint32)), f"colour differs in frame {k}"
         assert np.array_equal(g_depth.view(np.uint32), o_depth.view(np.uint32)), f"depth differs in frame {k}"
+
+
+def test_frame_dump_png(tmp_path):
+    """main.rs:693-715: RGBA32F frame -> RGBA8 (clamp, x255, round half away from zero) -> PNG."""
+    from test_outputs import read_png
+
+    tape = tape_of("spheres")
+    with T.State(tape, width=40, height=24, localsubdiv=5, voxel_capacity=1 << 18) as st:
+        st.reset()
+        st.update(0.0, T.look_at_inverse((2.2, 1.7, 2.6)))
+        st.set_inversion()
+        rgba, _ = st.trace()
+        want = np.floor(np.clip(rgba, 0.0, 1.0) * np.float32(255.0) + np.float32(0.5)).astype(np.uint8)
+        assert np.array_equal(st.frame_rgba8(), want)
+        st.save_png(tmp_path / "0.png")
+        assert np.array_equal(read_png(tmp_path / "0.png"), want)
