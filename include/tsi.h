@@ -191,10 +191,10 @@ typedef struct {
 int tsi_tape_info_get(tsi_ctx *ctx, tsi_tape_info *out);
 
 /* Compiles a tape's evaluation kernels into the code-object cache without a device or
- * context (no counterpart in the reference): a later tsi_upload_tape of the same tape then
- * only loads them.  Cache directory: $TSI_JIT_CACHE, else $XDG_CACHE_HOME/tsi-jit, else
+ * context (no counterpart in the reference): a later tsi_upload_tape of the same tape with the
+ * same pruning flags (TSI_FLAG_NO_PRUNE, TSI_FLAG_PRUNE_EXACT_ONLY) then only loads them.  Cache directory: $TSI_JIT_CACHE, else $XDG_CACHE_HOME/tsi-jit, else
  * ~/.cache/tsi-jit. */
-int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_info *out);
+int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t flags, tsi_tape_info *out);
 
 /* src/ir.rs IR::new + parseLLVMIR + to_gpu: builds the tape from .ll text holding
  * @_Z1fffff.  time is written into instruction 3 (ARGW).  *n receives the full

@@ -70,8 +70,9 @@ def timing_line(frame_number: int, rootfinding_ms_accumulated: float, frame_ms: 
             f"{_rust_f64_debug(rootfinding_ms_accumulated)}, actual_frame_time: {_rust_f64_debug(frame_ms)}\n")
 
 
-def precompile(tape) -> dict:
-    """Compiles a tape's evaluation kernels into the code-object cache (no GPU needed)."""
+def precompile(tape, prune=True) -> dict:
+    """Compiles a tape's evaluation kernels into the code-object cache (no GPU needed), for the
+    pruning setting a State will use (True, "exact" or False)."""
     if isinstance(tape, IR):
         tape = tape.gpu_instructions
     elif isinstance(tape, (str, bytes)):
@@ -79,7 +80,8 @@ def precompile(tape) -> dict:
     t = np.ascontiguousarray(tape, dtype=INSTR_DTYPE)
     buf = (GpuInstruction * len(t)).from_buffer_copy(t.tobytes())
     info = _lib.TapeInfo()
-    rc = lib().tsi_precompile_tape(buf, len(t), C.byref(info))
+    flags = _lib.TSI_FLAG_PRUNE_EXACT_ONLY if prune == "exact" else (0 if prune else _lib.TSI_FLAG_NO_PRUNE)
+    rc = lib().tsi_precompile_tape(buf, len(t), flags, C.byref(info))
     if rc != 0:
         raise TsiError(rc, "tsi_precompile_tape\n" + info.log.decode(errors="replace"))
     return {"compile_ms": info.compile_ms, "cache_hit": bool(info.cache_hit), "n": info.n}

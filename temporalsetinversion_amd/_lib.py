@@ -101,7 +101,7 @@ SIGNATURES = {
     "tsi_camera_zdist": (C.c_float, [C.c_uint32]),
     "tsi_tape_info_get": (C.c_int, [C.c_void_p, C.POINTER(TapeInfo)]),
     "tsi_last_trace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
-    "tsi_precompile_tape": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.POINTER(TapeInfo)]),
+    "tsi_precompile_tape": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.c_uint32, C.POINTER(TapeInfo)]),
     "tsi_strerror": (C.c_char_p, [C.c_int]),
 }
 

@@ -6,8 +6,9 @@
 //
 // Engine (evaluation of the tape for the wavefront's 64 cells):
 //   Engine(a, wave, lane)
-//   select(a, which, g, pruned_ok)  program for group g (pruned_ok: the group's cells lie
-//                                   inside their parents)
+//   select(a, which, g, pruned_ok, prec_in, rec_idx)  program for group g (pruned_ok: every
+//                                   parent has a liveness record and the group's cells lie inside
+//                                   their parents; the lane's parent record is prec_in + rec_idx)
 //   forward<PUSH>(a, x, y, z, t, stk, field, Rlo, Rhi) -> algorithmic f32 ops; PUSH saves the
 //                                   operands the reverse rules need on the wavefront's stack
 //                                   (stk: uniform base, row r of lane l at stk[64 r + l])
@@ -129,16 +130,17 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         // ---- program: the group's pruned program when every evaluated child lies inside its
         //      parent (box and t-window of the evaluation that produced the liveness record),
         //      else the full tape ----
+        // every parent of the group has a record (capacities are multiples of 8)
         bool pruned_ok = false;
-        if (prune && level > 0) {
-            const bool has = active && pidx < a.prune_cap;
-            const float wlo = has ? __uint_as_float(prec_in[pidx]) : 0.0f;
-            const float whi = has ? __uint_as_float(prec_in[a.prune_cap + pidx]) : 0.0f;
+        const uint32_t rec_idx = active ? pidx : g * 8u;
+        if (prune && level > 0 && g * 8u + 8u <= a.prune_cap) {
+            const float wlo = __uint_as_float(prec_in[rec_idx]);
+            const float whi = __uint_as_float(prec_in[a.prune_cap + rec_idx]);
             const bool inside = targ.lo >= wlo && targ.hi <= whi && cx.lo >= plo.x && cx.hi <= phi.x &&
                                 cy.lo >= plo.y && cy.hi <= phi.y && cz.lo >= plo.z && cz.hi <= phi.z;
             pruned_ok = ballot(guard && !inside) == 0;
         }
-        E.select(a, which, g, pruned_ok);
+        E.select(a, which, g, pruned_ok, prec_in, rec_idx);
 
         // ---- tape evaluation (wave-uniform) ----
         Iv field = I_FROM(0.0f);

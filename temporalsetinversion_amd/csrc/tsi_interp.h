@@ -23,12 +23,6 @@ typedef const __attribute__((address_space(4))) uint32_t *CWords;
 // 3 = undecided).  Full programs carry 0xFFFF.  A parent's decision holds in its children
 // (child box and t-window inside the parent's), so a decided MIN/MAX is a copy of its operand.
 TSI_DI uint32_t lane_mode(uint32_t w3, uint32_t ps) { return (w3 >> (16u + 2u * ps)) & 3u; }
-// MIN_II_RR / MAX_II_RR (compute.wgsl:2884-2903) as a choice: 1 left only, 2 right only, 3 both
-TSI_DI uint32_t min_choice(Iv l, Iv r) { return l.hi < r.lo ? 1u : (r.hi < l.lo ? 2u : 3u); }
-TSI_DI uint32_t max_choice(Iv l, Iv r) { return l.hi < r.lo ? 2u : (r.hi < l.lo ? 1u : 3u); }
-TSI_DI bool is_minmax(uint32_t form) {
-    return form == F_MIN_RR || form == F_MAX_RR || form == F_MIN_RL || form == F_MAX_RL;
-}
 
 // Per-lane record of the MIN/MAX choices made by a forward pass (2 bits per MIN/MAX record,
 // 16 per word), kept in a per-wave buffer column for the liveness walk.
@@ -365,7 +359,7 @@ struct InterpEngine {
         ch.n = 0u;
     }
 
-    TSI_DI void select(const KArgs &a, int which, uint32_t g, bool pruned_ok) {
+    TSI_DI void select(const KArgs &a, int which, uint32_t g, bool pruned_ok, const uint32_t *, uint32_t) {
         fprog = a.fprog;
         rprog = a.rprog;
         n = a.nfwd;

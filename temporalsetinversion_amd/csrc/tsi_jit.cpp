@@ -77,55 +77,141 @@ const char *kClose = "        }\n    }\n";
 
 }  // namespace
 
+bool is_minmax_form(uint32_t form) {
+    return form == F_MIN_RR || form == F_MAX_RR || form == F_MIN_RL || form == F_MAX_RL;
+}
+
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L) {
+    JitLayout y;
+    y.group = region_records();
+    y.regions = (L.nfwd + y.group - 1) / y.group;
+    y.region_words = (y.regions + 31) / 32;
+    for (uint32_t k = 0; k < L.nfwd; ++k) y.minmax += is_minmax_form(fwd[k].form) ? 1u : 0u;
+    y.mode_words = (y.minmax + 15) / 16;
+    y.record_words = 2 + y.region_words + y.mode_words;
+    return y;
+}
+
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
-                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel) {
+                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune) {
     std::string o;
-    o.reserve(size_t(n) * 400u);
-    const uint32_t kGroup = region_records();
+    o.reserve(size_t(n) * 500u);
+    const JitLayout Y = jit_layout(fwd, L);
+    const uint32_t G = Y.group, RW = Y.region_words, MW = Y.mode_words;
+    auto region_of = [&](uint32_t j) { return (j - 4) / G; };
+    // MIN/MAX index of record j (records in order)
+    std::vector<int> mmx(n, -1);
+    for (uint32_t j = 4, m = 0; j < n; ++j)
+        if (is_minmax_form(fwd[j - 4].form)) mmx[j] = int(m++);
+    auto bit = [&](uint32_t r) {   // uniform test of region r's bit in the wave's region mask
+        return prune ? "((rm" + std::to_string(r / 32) + " >> " + std::to_string(r % 32) + ") & 1u)"
+                     : std::string("go");
+    };
+    auto open_region = [&](uint32_t r) {
+        if (prune) return "    if (" + bit(r) + ") {\n";
+        return std::string("    {\n        uint32_t go;\n        asm volatile(\"s_mov_b32 %0, 1\" : \"=s\"(go));\n"
+                           "        if (go) {\n");
+    };
+    auto close_region = [&]() { return prune ? std::string("    }\n") : std::string("        }\n    }\n"); };
+    auto mode_of = [&](uint32_t j) {
+        const int m = mmx[j];
+        return "((mw" + std::to_string(m / 16) + " >> " + std::to_string(2 * (m % 16)) + "u) & 3u)";
+    };
+
     o += "// generated by tsi_jit.cpp: straight-line evaluation of one tape\n";
     o += "#include \"tsi_eval.h\"\n#include \"tsi_rules.h\"\n\nnamespace tsi {\n";
     o += "static constexpr uint32_t kTapeRecords = " + std::to_string(L.nfwd) + "u;\n";
-    o += "static constexpr uint32_t kTapeFwdCost = " + std::to_string(L.fwd_cost) + "u;\n\n";
+    o += "static constexpr uint32_t kRegionWords = " + std::to_string(RW) + "u, kModeWords = " + std::to_string(MW) +
+         "u;\n";
+    o += "struct RegionMask { uint32_t w[" + std::to_string(RW) + "]; };\n";
+    o += "struct FwdOut { Iv field; uint32_t cost, records; };\n";
+    o += "struct RevOut { Iv s0, s1, s2, s3; uint32_t ops; };\n\n";
+
+    // common prologue of the pruned functions: uniform region words, this lane's parent modes
+    auto prologue = [&](bool need_modes) {
+        std::string p;
+        if (!prune) return p;
+        for (uint32_t w = 0; w < RW; ++w)
+            p += "    const uint32_t rm" + std::to_string(w) + " = __builtin_amdgcn_readfirstlane(mask.w[" +
+                 std::to_string(w) + "]);\n";
+        if (need_modes)
+            for (uint32_t w = 0; w < MW; ++w)
+                p += "    const uint32_t mw" + std::to_string(w) + " = prec ? prec[size_t(" + std::to_string(2 + RW + w) +
+                     "u) * cap] : 0xFFFFFFFFu;\n";
+        return p;
+    };
 
     // ---- forward (compute.wgsl:3633-3772) ----
     // The tape functions are calls, not inlined into the level kernel: inlined, the register
     // allocation of the kernel's own state and of the tape interfere (several times the
     // registers of either alone, with spills); a call costs a few instructions per wavefront.
-    o += "template <bool PUSH>\n__device__ __noinline__ Iv tape_forward(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, uint32_t lane) {\n";
-    o += "    const Iv ZI = I_FROM(0.0f);\n";
-    for (uint32_t j = 4; j < n; ++j) o += "    Iv " + V(j) + ";\n";
+    o += "template <bool PUSH>\n__device__ __noinline__ FwdOut tape_forward(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
+         "uint32_t lane, const uint32_t *prec, uint32_t cap, uint32_t *chbuf, RegionMask mask) {\n";
+    o += "    const Iv ZI = I_FROM(0.0f);\n    uint32_t cost = 0, recs = 0;\n";
+    o += prologue(true);
+    if (prune)
+        for (uint32_t w = 0; w < MW; ++w) o += "    uint32_t cw" + std::to_string(w) + " = 0xFFFFFFFFu;\n";
+    for (uint32_t j = 4; j < n; ++j) o += "    Iv " + V(j) + (prune ? " = ZI;\n" : ";\n");
     for (uint32_t j = 4; j < n; ++j) {
         const auto &g = tape[j];
         const FwdIns &f = fwd[j - 4];
         const uint32_t side = side_of(g);
-        if ((j - 4) % kGroup == 0) o += kOpen;
+        const uint32_t r = region_of(j);
+        if ((j - 4) % G == 0) {
+            o += open_region(r);
+            uint32_t cost = 0, recs = 0;
+            for (uint32_t k = j; k < n && region_of(k) == r; ++k, ++recs) cost += fwd[k - 4].cost;
+            o += "        cost += " + std::to_string(cost) + "u; recs += " + std::to_string(recs) + "u;\n";
+        }
         const std::string l = (side == 0 || side == 1) ? V(larg_of(g)) : "ZI";
-        const std::string r = (side == 0 || side == 2) ? V(rarg_of(g)) : "ZI";
-        o += "        " + V(j) + " = fix(fwd_form_t<true>(" + std::to_string(f.form) + "u, " + l + ", " + r + ", " +
-             lit(g.literal) + "));\n";
+        const std::string rr = (side == 0 || side == 2) ? V(rarg_of(g)) : "ZI";
+        const std::string c = lit(g.literal);
+        const std::string form = std::to_string(f.form) + "u";
+        if (prune && mmx[j] >= 0) {
+            // the parent's decided choice makes the MIN/MAX a copy (tsi_interp.h fwd_step)
+            const std::string J = std::to_string(j);
+            const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
+            const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
+            const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
+            o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
+            o += "        " + V(j) + " = fix(sel(md" + J + " == 1u, " + l + ", sel(md" + J + " == 2u, " + rhs +
+                 ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "))));\n";
+            const int m = mmx[j];
+            const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
+            o += "        { const uint32_t ch = md" + J + " != 3u ? md" + J + " : " +
+                 (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ");\n";
+            o += "          cw" + w + " = (cw" + w + " & ~(3u << " + sh + ")) | (ch << " + sh + "); }\n";
+        } else {
+            o += "        " + V(j) + " = fix(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "));\n";
+        }
         const uint32_t npush = f.npush_side & 3u;
         if (npush) {
             o += "        if (PUSH) { ";
             if (npush == 3) {
-                o += P(f.push) + " = make_float2(" + r + ".lo, " + r + ".hi); ";
+                o += P(f.push) + " = make_float2(" + rr + ".lo, " + rr + ".hi); ";
             } else {
                 o += P(f.push) + " = make_float2(" + l + ".lo, " + l + ".hi); ";
-                if (npush == 2) o += P(f.push + 1u) + " = make_float2(" + r + ".lo, " + r + ".hi); ";
+                if (npush == 2) o += P(f.push + 1u) + " = make_float2(" + rr + ".lo, " + rr + ".hi); ";
             }
             o += "}\n";
         }
-        if ((j - 4) % kGroup == kGroup - 1 || j == n - 1) o += kClose;
+        if ((j - 4) % G == G - 1 || j == n - 1) o += close_region();
     }
-    o += "    return " + V(n - 1) + ";\n}\n\n";
+    if (prune)
+        for (uint32_t w = 0; w < MW; ++w)
+            o += "    chbuf[" + std::to_string(w * 64u) + "u] = cw" + std::to_string(w) + ";\n";
+    o += "    return FwdOut{" + V(n - 1) + ", cost, recs};\n}\n\n";
 
     // ---- reverse (compute.wgsl:3780-3964) ----
-    // sens of instruction k is s<k> from its first contribution on (j descending); before
-    // that it reads [+0, +0].  Temporaries of record j carry the suffix _j.
+    // sens of instruction k is s<k>, [+0, +0] until its first contribution (j descending).
+    // Temporaries of record j carry the suffix _j.  Decided MIN/MAX pass the contribution to
+    // the kept operand only (tsi_interp.h rev_step).
     if (rev_kernel) {
-        o += "__device__ __noinline__ uint32_t tape_reverse(const float2 *stk, uint32_t lane, Iv sens[4]) {\n";
+        o += "__device__ __noinline__ RevOut tape_reverse(const float2 *stk, uint32_t lane, const uint32_t *prec, "
+             "uint32_t cap, RegionMask mask) {\n";
         o += "    const Iv ZI = I_FROM(0.0f), ONE = I_FROM(1.0f);\n    uint32_t ops = 0;\n";
+        o += prologue(true);
         std::vector<char> touched(n, 0), live(n, 0);
-        // declarations: every sens that receives a contribution, every saved operand
         for (uint32_t j = 4; j < n; ++j) {
             const auto &g = tape[j];
             const uint32_t side = side_of(g);
@@ -136,10 +222,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (side != 3 && npush == 2) o += "    float2 q_" + std::to_string(j) + ";\n";
         }
         for (uint32_t k = 0; k < n; ++k)
-            if (touched[k]) o += "    Iv " + S(k) + ";\n";
+            if (touched[k]) o += "    Iv " + S(k) + (prune ? " = ZI;\n" : ";\n");
         uint64_t static_ops = 0;
-        // loads of the saved operands of records hi-1 .. lo
-        auto emit_loads = [&](uint32_t hi, uint32_t lo) {
+        auto emit_loads = [&](uint32_t hi, uint32_t lo) {   // saved operands of records hi-1 .. lo
             for (uint32_t jj = hi; jj-- > lo;) {
                 if (side_of(tape[jj]) == 3) continue;
                 const FwdIns &f = fwd[jj - 4];
@@ -149,18 +234,31 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 if (npush == 2) o += "        q_" + J + " = " + P(f.push + 1u) + ";\n";
             }
         };
-        auto region_lo = [&](uint32_t hi) { return hi - 4u > kGroup ? hi - kGroup : 4u; };
-        o += kOpen;
-        emit_loads(n, region_lo(n));
-        o += kClose;
+        // regions from the top: records [hi, lo) with lo the region start
+        auto region_lo = [&](uint32_t hi) { return 4u + ((hi - 5u) / G) * G; };
+        if (!prune) {
+            o += open_region(0);
+            emit_loads(n, region_lo(n));
+            o += close_region();
+        }
         for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) {
-            const uint32_t lo = region_lo(hi);
-            o += kOpen;
+            const uint32_t lo = region_lo(hi), r = region_of(lo);
+            o += open_region(r);
+            if (prune) {
+                // operands of this region: loaded here unless the region above (run before it)
+                // prefetched them
+                if (hi == n) emit_loads(hi, lo);
+                else {
+                    o += "        if (!" + bit(region_of(hi)) + ") {\n";
+                    emit_loads(hi, lo);
+                    o += "        }\n";
+                }
+            }
             if (lo > 4u) emit_loads(lo, region_lo(lo));   // the next region's operands go in flight
             for (uint32_t j = hi - 1; j >= lo; --j) {
                 const auto &g = tape[j];
                 const FwdIns &f = fwd[j - 4];
-                const RevIns &r = rev[(n - 1) - j];
+                const RevIns &rv = rev[(n - 1) - j];
                 const uint32_t side = side_of(g);
                 if (side != 3) {
                     const uint32_t a = larg_of(g), b = rarg_of(g);
@@ -172,36 +270,53 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     const uint32_t npush = f.npush_side & 3u;
                     const std::string x = npush ? "I(p" + J + ".x, p" + J + ".y)" : "ZI";
                     const std::string y = npush == 2 ? "I(q" + J + ".x, q" + J + ".y)" : "ZI";
-                    o += "        const Iv2 d" + J + " = rev_form_t<true>(" + std::to_string(f.form) + "u, " + x + ", " + y +
-                         ", " + lit(g.literal) + ");\n";
-                    const std::string cost = std::to_string(unsigned(r.cost)) + "u";
+                    const std::string form = std::to_string(f.form) + "u";
+                    const bool mm = prune && mmx[j] >= 0;
+                    if (mm) {
+                        o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
+                        o += "        Iv2 d" + J + " = rev_form_t<true>(" + form + ", " + x + ", " + y + ", " +
+                             lit(g.literal) + ");\n";
+                        // decided: the kept operand receives [1, 1] (rev_step)
+                        o += "        d" + J + ".a = sel(md" + J + " != 3u, ONE, d" + J + ".a);\n";
+                        o += "        d" + J + ".b = sel(md" + J + " != 3u, ONE, d" + J + ".b);\n";
+                    } else {
+                        o += "        const Iv2 d" + J + " = rev_form_t<true>(" + form + ", " + x + ", " + y + ", " +
+                             lit(g.literal) + ");\n";
+                    }
+                    const std::string cost = std::to_string(unsigned(rv.cost)) + "u";
                     auto contrib = [&](uint32_t k, const char *dx) {
                         return "ADD_II(MUL_II(fix(d" + J + "." + dx + "), " + sj + "), " +
                                (live[k] ? S(k) : std::string("ZI")) + ")";
                     };
+                    auto old = [&](uint32_t k) { return live[k] ? S(k) : std::string("ZI"); };
                     if (side == 1 || side == 2) {
                         const uint32_t k = side == 1 ? a : b;
-                        const bool u1 = side == 1 ? (r.flags & R_U1_A) != 0 : (r.flags & R_U1_B) != 0;
+                        const bool u1 = side == 1 ? (rv.flags & R_U1_A) != 0 : (rv.flags & R_U1_B) != 0;
                         std::string val = contrib(k, "a");
+                        if (mm) val = "((md" + J + " & 1u) ? " + val + " : " + old(k) + ")";
                         if (u1) {
                             o += "        const bool z" + J + " = " + sj0 + ";\n";
                             o += "        ops += z" + J + " ? 0u : " + cost + ";\n";
                             val = "sel(z" + J + ", ZI, " + val + ")";
                         } else {
-                            static_ops += r.cost;
+                            static_ops += rv.cost;
                         }
                         o += "        " + S(k) + " = " + val + ";\n";
                         live[k] = 1;
                     } else {   // side 0: both previous values are read before either is written
-                        const bool u1 = (r.flags & R_U1_A) && (r.flags & R_U1_B);
+                        const bool u1 = (rv.flags & R_U1_A) && (rv.flags & R_U1_B);
                         std::string va = contrib(a, "a"), vb = contrib(b, "b");
+                        if (mm) {
+                            va = "((md" + J + " & 1u) ? " + va + " : " + old(a) + ")";
+                            vb = "((md" + J + " >> 1) ? " + vb + " : " + old(b) + ")";
+                        }
                         if (u1) {
                             o += "        const bool z" + J + " = " + sj0 + ";\n";
                             o += "        ops += z" + J + " ? 0u : " + cost + ";\n";
                             va = "sel(z" + J + ", ZI, " + va + ")";
                             vb = "sel(z" + J + ", ZI, " + vb + ")";
                         } else {
-                            static_ops += r.cost;
+                            static_ops += rv.cost;
                         }
                         o += "        const Iv na" + J + " = " + va + ";\n";
                         o += "        const Iv nb" + J + " = " + vb + ";\n";
@@ -212,35 +327,114 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 }
                 if (j == lo) break;
             }
-            o += kClose;
+            // work of the records that cannot be short-circuited, counted when the region runs
+            if (static_ops) o += "        ops += " + std::to_string(static_ops) + "u;\n";
+            static_ops = 0;
+            o += close_region();
         }
-        for (uint32_t k = 0; k < 4; ++k)
-            o += "    sens[" + std::to_string(k) + "] = " + (live[k] ? S(k) : std::string("ZI")) + ";\n";
-        o += "    return ops + " + std::to_string(static_ops) + "u;\n}\n\n";
+        o += "    return RevOut{";
+        for (uint32_t k = 0; k < 4; ++k) o += (live[k] ? S(k) : std::string("ZI")) + ", ";
+        o += "ops};\n}\n\n";
     }
 
-    o += "struct TapeEngine {\n"
-         "    static constexpr bool kPrune = false;\n"
-         "    uint32_t lane;\n"
-         "    TSI_DI TapeEngine(const KArgs &, uint32_t, uint32_t l) : lane(l) {}\n"
-         "    TSI_DI void select(const KArgs &, int, uint32_t, bool) {}\n"
-         "    template <bool PUSH>\n"
-         "    TSI_DI uint32_t forward(const KArgs &, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n"
-         "        field = tape_forward<PUSH>(x, y, z, t, stk, lane);\n"
-         "        return kTapeFwdCost;\n"
-         "    }\n";
+    // ---- liveness (row P): the records reachable from the output through this cell's own
+    // MIN/MAX choices (compute.wgsl:2884-2903), as region bits and per-MIN/MAX modes for the
+    // children's evaluation.  Regions skipped for the parent cannot hold live records.
+    if (prune) {
+        o += "__device__ __noinline__ void tape_liveness(const uint32_t *chbuf, uint32_t *rec, uint32_t cap, "
+             "RegionMask mask) {\n";
+        o += prologue(false);
+        for (uint32_t w = 0; w < MW; ++w) {
+            o += "    const uint32_t cw" + std::to_string(w) + " = chbuf[" + std::to_string(w * 64u) + "u];\n";
+            o += "    uint32_t ow" + std::to_string(w) + " = 0xFFFFFFFFu;\n";
+        }
+        for (uint32_t k = 0; k < n; ++k) o += "    bool l" + std::to_string(k) + " = " + (k == n - 1 ? "true" : "false") + ";\n";
+        for (uint32_t w = 0; w < RW; ++w) o += "    uint32_t rw" + std::to_string(w) + " = 0u;\n";
+        auto region_lo = [&](uint32_t hi) { return 4u + ((hi - 5u) / G) * G; };
+        for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) {
+            const uint32_t lo = region_lo(hi), r = region_of(lo);
+            o += open_region(r);
+            const std::string gr = "g" + std::to_string(r);
+            o += "        bool " + gr + " = false;\n";
+            for (uint32_t j = hi - 1; j >= lo; --j) {
+                const auto &g = tape[j];
+                const uint32_t side = side_of(g);
+                const std::string lj = "l" + std::to_string(j);
+                o += "        " + gr + " = " + gr + " || " + lj + ";\n";
+                if (mmx[j] >= 0) {
+                    const int m = mmx[j];
+                    const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
+                    const std::string ch = "((cw" + w + " >> " + sh + ") & 3u)";
+                    o += "        ow" + w + " = (ow" + w + " & ~(3u << " + sh + ")) | ((" + lj + " ? " + ch + " : 3u) << " +
+                         sh + ");\n";
+                    if (side == 0 || side == 1) {
+                        const std::string la = "l" + std::to_string(larg_of(g));
+                        o += "        " + la + " = " + la + " || (" + lj + " && (" + ch + " & 1u));\n";
+                    }
+                    if (side == 0 || side == 2) {
+                        const std::string lb = "l" + std::to_string(rarg_of(g));
+                        o += "        " + lb + " = " + lb + " || (" + lj + " && (" + ch + " >> 1));\n";
+                    }
+                } else {
+                    if (side == 0 || side == 1) {
+                        const std::string la = "l" + std::to_string(larg_of(g));
+                        o += "        " + la + " = " + la + " || " + lj + ";\n";
+                    }
+                    if (side == 0 || side == 2) {
+                        const std::string lb = "l" + std::to_string(rarg_of(g));
+                        o += "        " + lb + " = " + lb + " || " + lj + ";\n";
+                    }
+                }
+                if (j == lo) break;
+            }
+            o += "        rw" + std::to_string(r / 32) + " |= " + gr + " ? " + std::to_string(1u << (r % 32)) + "u : 0u;\n";
+            o += close_region();
+        }
+        o += "    if (rec) {\n";
+        for (uint32_t w = 0; w < RW; ++w)
+            o += "        rec[size_t(" + std::to_string(2 + w) + "u) * cap] = rw" + std::to_string(w) + ";\n";
+        for (uint32_t w = 0; w < MW; ++w)
+            o += "        rec[size_t(" + std::to_string(2 + RW + w) + "u) * cap] = ow" + std::to_string(w) + ";\n";
+        o += "    }\n}\n\n";
+    }
+
+    // ---- the evaluation engine of eval_level (tsi_eval.h) ----
+    o += std::string("struct TapeEngine {\n    static constexpr bool kPrune = ") + (prune ? "true" : "false") + ";\n";
+    o += "    uint32_t lane, recs;\n    const uint32_t *prec;\n    uint32_t *chbuf;\n    RegionMask mask;\n";
+    o += "    TSI_DI TapeEngine(const KArgs &a, uint32_t wave, uint32_t l) : lane(l), recs(kTapeRecords), prec(nullptr) {\n"
+         "        for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n"
+         "        chbuf = a.choice ? a.choice + size_t(wave) * kModeWords * 64u + l : nullptr;\n    }\n";
+    o += "    TSI_DI void select(const KArgs &a, int, uint32_t, bool pruned_ok, const uint32_t *prec_in, uint32_t pidx) {\n"
+         "        prec = nullptr;\n"
+         "        for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n";
+    if (prune)
+        o += "        if (pruned_ok) {\n"
+             "            prec = prec_in + pidx;\n"
+             "            for (uint32_t w = 0; w < kRegionWords; ++w) {\n"
+             "                uint32_t v = prec[size_t(2u + w) * a.prune_cap];\n"
+             "                v |= uint32_t(__shfl_xor(int(v), 8, 64));\n"
+             "                v |= uint32_t(__shfl_xor(int(v), 16, 64));\n"
+             "                v |= uint32_t(__shfl_xor(int(v), 32, 64));\n"
+             "                mask.w[w] = v;\n"
+             "            }\n        }\n";
+    o += "    }\n";
+    o += "    template <bool PUSH>\n"
+         "    TSI_DI uint32_t forward(const KArgs &a, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n"
+         "        const FwdOut r = tape_forward<PUSH>(x, y, z, t, stk, lane, prec, a.prune_cap, chbuf, mask);\n"
+         "        field = r.field;\n        recs = r.records;\n        return r.cost;\n    }\n";
     if (rev_kernel)
-        o += "    TSI_DI uint32_t reverse(const KArgs &, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n"
-             "        return tape_reverse(stk, lane, sens);\n"
-             "    }\n";
+        o += "    TSI_DI uint32_t reverse(const KArgs &a, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n"
+             "        const RevOut r = tape_reverse(stk, lane, prec, a.prune_cap, mask);\n"
+             "        sens[0] = r.s0; sens[1] = r.s1; sens[2] = r.s2; sens[3] = r.s3;\n        return r.ops;\n    }\n";
     else
         o += "    TSI_DI uint32_t reverse(const KArgs &, const float2 *, Iv sens[4], Slots &, Slots &) {\n"
-             "        for (int k = 0; k < 4; ++k) sens[k] = I_FROM(0.0f);\n"
-             "        return 0u;\n"
-             "    }\n";
-    o += "    TSI_DI uint32_t records() const { return kTapeRecords; }\n"
-         "    TSI_DI void liveness(const KArgs &, uint32_t *) {}\n"
-         "};\n\n}  // namespace tsi\n\n";
+             "        for (int k = 0; k < 4; ++k) sens[k] = I_FROM(0.0f);\n        return 0u;\n    }\n";
+    o += "    TSI_DI uint32_t records() const { return recs; }\n";
+    if (prune)
+        o += "    TSI_DI void liveness(const KArgs &a, uint32_t *rec) { tape_liveness(chbuf, rec, a.prune_cap, mask); }\n";
+    else
+        o += "    TSI_DI void liveness(const KArgs &, uint32_t *) {}\n";
+    o += "};\n\n}  // namespace tsi\n\n";
     o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") +
          (rev_kernel ? "tsi_tape_eval_rev" : "tsi_tape_eval_fwd") +
          "(tsi::KArgs a, int level, float time) {\n    tsi::eval_level<" + (rev_kernel ? "true" : "false") +

@@ -8,10 +8,20 @@
 
 namespace tsi {
 
+// Layout of the per-cell liveness records the pruned straight-line kernels exchange between
+// levels: t-window lo/hi, region_words words of region bits (one bit per `group` records),
+// mode_words words of MIN/MAX choices (2 bits each, 16 per word).
+struct JitLayout {
+    uint32_t group = 16, regions = 0, region_words = 0, minmax = 0, mode_words = 0, record_words = 0;
+};
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L);
+
 // HIP source of the level kernel for one tape: tsi_tape_eval_fwd (forward-only frontier list)
-// or tsi_tape_eval_rev (forward with saved operands, then reverse).
+// or tsi_tape_eval_rev (forward with saved operands, then reverse).  prune: regions dead for
+// all of a wavefront's parents are skipped and decided MIN/MAX become copies (liveness records
+// written for enqueued cells).
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
-                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel);
+                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune);
 
 // Compiles `src` for gfx950 with hiprtc (the library's kernel headers are embedded), or finds
 // the code object in this process's cache or the on-disk cache ($TSI_JIT_CACHE, default

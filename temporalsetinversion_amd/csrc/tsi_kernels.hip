@@ -419,11 +419,12 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a);
     if (e_eval0) hipEventRecord(e_eval0, s);
     for (int d = 0; d < fl.levels; ++d) {
-        if (a.prune && d > 0) {
+        const bool compiled = fl.tape_eval[0] && fl.tape_eval[1];
+        if (a.prune && d > 0 && !compiled) {   // the interpreter's group programs
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 0);
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 1);
         }
-        if (fl.tape_eval[0] && fl.tape_eval[1]) {
+        if (compiled) {
             KArgs ka = a;
             int level = d;
             float t = fl.time;

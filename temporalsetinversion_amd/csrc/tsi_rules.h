@@ -58,6 +58,13 @@ TSI_DI Iv2 rare_reverse_i(uint32_t form, Iv l, Iv r, float c) {
 }
 __device__ __noinline__ Iv2 rare_reverse(uint32_t form, Iv l, Iv r, float c) { return rare_reverse_i(form, l, r, c); }
 
+// MIN_II_RR / MAX_II_RR (compute.wgsl:2884-2903) as a choice: 1 left only, 2 right only, 3 both
+TSI_DI uint32_t min_choice(Iv l, Iv r) { return l.hi < r.lo ? 1u : (r.hi < l.lo ? 2u : 3u); }
+TSI_DI uint32_t max_choice(Iv l, Iv r) { return l.hi < r.lo ? 2u : (r.hi < l.lo ? 1u : 3u); }
+TSI_DI bool is_minmax(uint32_t form) {
+    return form == F_MIN_RR || form == F_MAX_RR || form == F_MIN_RL || form == F_MAX_RL;
+}
+
 TSI_DI bool is_rare(uint32_t form) {
     switch (form) {
     case F_SIN: case F_COS: case F_EXP: case F_LOG: case F_FLOOR: case F_POW_RR: case F_POW_RL:

@@ -149,20 +149,30 @@ void release_prune(tsi_ctx *c) {
     c->prune_cap = c->nchunks = c->choice_len = c->pool_cap = 0;
 }
 
-// Buffers of the pruned programs for a lowered tape: liveness records for prune_cap entries
-// of each frontier list (two level parities), group descriptors, per-wavefront MIN/MAX choice
-// columns, and the program pool (each level reuses it from offset 0).
-int alloc_prune(tsi_ctx *c, const Lowered &L) {
+// Whether a tape is evaluated with pruning under these flags (tsi.h TSI_FLAG_*).
+bool prune_wanted(uint32_t flags, const Lowered &L) {
+    if (L.prune_class == kPruneOff || (flags & TSI_FLAG_NO_PRUNE)) return false;
+    return !(L.prune_class == kPruneApprox && (flags & TSI_FLAG_PRUNE_EXACT_ONLY));
+}
+
+// Buffers of pruned evaluation for a lowered tape: liveness records for prune_cap entries of
+// each frontier list (two level parities) and per-wavefront MIN/MAX choice columns; for the
+// interpreter also group descriptors and the program pool (each level reuses it from 0).
+int alloc_prune(tsi_ctx *c, const Lowered &L, const std::vector<FwdIns> &fwd) {
     release_prune(c);
-    if (L.prune_class == kPruneOff || (c->p.flags & TSI_FLAG_NO_PRUNE)) return TSI_OK;
-    if (!(c->p.flags & TSI_FLAG_INTERPRET)) return TSI_OK;   // compiled kernels run the full tape
-    if (L.prune_class == kPruneApprox && (c->p.flags & TSI_FLAG_PRUNE_EXACT_ONLY)) return TSI_OK;
+    if (!prune_wanted(c->p.flags, L)) return TSI_OK;
+    const bool interp = (c->p.flags & TSI_FLAG_INTERPRET) != 0;
     uint64_t cap = c->p.prune_capacity ? c->p.prune_capacity : (uint64_t(1) << 19);
     if (cap > c->list_cap) cap = c->list_cap;
     cap = (cap + 7) / 8 * 8;
     const uint32_t nchunks = (L.nfwd + 31) / 32;
-    const uint64_t words = uint64_t(kRecHead + 3 * nchunks) * cap;
-    const uint32_t choice_len = (L.nminmax + 15) / 16 ? (L.nminmax + 15) / 16 : 1;
+    uint64_t words = uint64_t(kRecHead + 3 * nchunks) * cap;
+    uint32_t choice_len = (L.nminmax + 15) / 16 ? (L.nminmax + 15) / 16 : 1;
+    if (!interp) {   // the compiled kernels' record layout (tsi_jit.h)
+        const JitLayout Y = jit_layout(fwd, L);
+        words = uint64_t(Y.record_words) * cap;
+        choice_len = Y.mode_words ? Y.mode_words : 1;
+    }
     const uint32_t waves = (c->grid_fwd > c->grid_rev ? c->grid_fwd : c->grid_rev) * 4u;
     // worst case: every group of every list keeps the whole program, twice (fwd + rev)
     const uint64_t per_group = 2ull * ((L.nfwd + 3) / 4 * 4 + 8);
@@ -173,12 +183,16 @@ int alloc_prune(tsi_ctx *c, const Lowered &L) {
     for (auto &l : c->prune_rec)
         for (auto &x : l)
             if ((rc = dmalloc(&x, words))) { release_prune(c); return rc; }
-    if ((rc = dmalloc(&c->gdesc[0], 2 * (c->list_cap / 8 + 1))) ||
-        (rc = dmalloc(&c->gdesc[1], 2 * (c->list_cap / 8 + 1))) ||
-        (rc = dmalloc(&c->choice, size_t(waves) * 64u * choice_len)) || (rc = dmalloc(&c->pool, pool))) {
+    if ((rc = dmalloc(&c->choice, size_t(waves) * 64u * choice_len))) {
         release_prune(c);
         return rc;
     }
+    if (interp && ((rc = dmalloc(&c->gdesc[0], 2 * (c->list_cap / 8 + 1))) ||
+                   (rc = dmalloc(&c->gdesc[1], 2 * (c->list_cap / 8 + 1))) || (rc = dmalloc(&c->pool, pool)))) {
+        release_prune(c);
+        return rc;
+    }
+    if (!interp) pool = 0;
     c->prune_cap = uint32_t(cap);
     c->nchunks = nchunks;
     c->choice_len = choice_len;
@@ -205,7 +219,7 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
     double ms[2] = {0.0, 0.0};
     int rc[2] = {TSI_OK, TSI_OK};
     try {
-        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1);
+        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, c->prune);
         // the two kernels compile in parallel
         std::thread t([&] { rc[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
         rc[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
@@ -391,7 +405,7 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
         if ((rc = dmalloc(&c->stack, need / sizeof(float2)))) return rc;
         c->stack_bytes = need;
     }
-    if ((rc = alloc_prune(c, L))) return rc;
+    if ((rc = alloc_prune(c, L, fwd))) return rc;
     std::memset(&c->info, 0, sizeof c->info);
     c->info.n = n;
     c->info.stack = L.stack;
@@ -465,7 +479,7 @@ int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         stats->enqueued = h.stats[kStatEnqueued];
         stats->allocated_nodes = h.stats[kStatLive] * 8u;
         stats->eval_launches = uint64_t(2 * c->p.localsubdiv);
-        stats->prune_launches = c->prune ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
+        stats->prune_launches = c->prune && !c->eval_fn[0] ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
     }
     c->used_nodes = (uint64_t(h.bump) + 1u) * 8u;
     if (c->used_nodes > c->nodes) c->used_nodes = c->nodes;
@@ -619,7 +633,7 @@ int tsi_read_tape(tsi_ctx *c, tsi_gpu_instruction *out, uint32_t cap, uint32_t *
     return TSI_OK;
 }
 
-int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_info *out) {
+int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t flags, tsi_tape_info *out) {
     if (!tape) return TSI_E_ARG;
     if (n > 4096) return TSI_E_TAPE_TOO_LONG;
     std::vector<FwdIns> fwd;
@@ -640,7 +654,7 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, tsi_tape_in
             bool hit[2] = {false, false};
             double ms[2] = {0.0, 0.0};
             int r[2] = {TSI_OK, TSI_OK};
-            for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1);
+            for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, prune_wanted(flags, L));
             std::thread t([&] { r[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
             r[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
             t.join();

@@ -16,6 +16,7 @@ SEQ = [k / 60.0 for k in range(0, 8)] + [0.5, 0.75, 1.0]
 # evaluation engines: the tape compiled into straight-line kernels (default), and the
 # interpreter with and without pruned per-group programs
 ENGINES = {"compiled": dict(engine="compiled"),
+           "compiled-full": dict(engine="compiled", prune=False),
            "interp-pruned": dict(engine="interpreter", prune=True),
            "interp-full": dict(engine="interpreter", prune=False)}
 
@@ -41,7 +42,7 @@ def test_param_variants(opts, engine):
         assert_same_nodes(g, o, f"{opts} frame {k}")
 
 
-@pytest.mark.parametrize("engine", ["compiled", "interp-full"])
+@pytest.mark.parametrize("engine", ["compiled-full", "interp-full"])
 def test_stats_match_oracle(engine):
     tape = tape_of("spheres")
     for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:5], localsubdiv=6, **ENGINES[engine]):
@@ -54,12 +55,13 @@ def test_stats_match_oracle(engine):
         assert s["pruned_instr"] == 0
 
 
+@pytest.mark.parametrize("engine", ["compiled", "interp-pruned"])
 @pytest.mark.parametrize("scene,L", [("spheres_linear", 6), ("capsules", 5)])
-def test_pruning_prunes(scene, L):
-    """Pruned programs run fewer records than the full tape and the same cells as the oracle."""
+def test_pruning_prunes(scene, L, engine):
+    """Pruned evaluation runs fewer records than the full tape and the same cells as the oracle."""
     tape = tape_of(scene)
     nfwd = len(tape) - 4
-    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:4], localsubdiv=L, **ENGINES["interp-pruned"]):
+    for k, g, o, gs, os_, _, _ in run_pair(tape, SEQ[:4], localsubdiv=L, **ENGINES[engine]):
         assert gs["cell_evals"] == os_.cell_evals
         assert gs["rev_evals"] == os_.rev_evals
         assert gs["enqueued"] == os_.enqueued
