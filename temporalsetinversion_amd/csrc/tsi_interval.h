@@ -32,6 +32,8 @@ struct Iv { float lo, hi; };
 
 TSI_DI float wmin(float a, float b) { return __builtin_fminimum_numf(a, b); }
 TSI_DI float wmax(float a, float b) { return __builtin_fmaximum_numf(a, b); }
+TSI_DI float wmin4(float x, float y, float z, float w) { return wmin(wmin(x, y), wmin(z, w)); }
+TSI_DI float wmax4(float x, float y, float z, float w) { return wmax(wmax(x, y), wmax(z, w)); }
 TSI_DI Iv I(float lo, float hi) { return Iv{lo, hi}; }
 TSI_DI Iv I_FROM(float a) { return Iv{a, a}; }
 TSI_DI bool is_nan_w(float a) { return !(a < 0.0f || a > 0.0f || a == 0.0f); }
@@ -189,7 +191,7 @@ TSI_DI Iv NEG_I(Iv a) { return I(-a.hi, -a.lo); }                               
 
 TSI_DI Iv MUL_II(Iv a, Iv b) {                                                           // 2555
     float x = a.lo * b.lo, y = a.lo * b.hi, z = a.hi * b.lo, w = a.hi * b.hi;
-    return I(wmin(wmin(x, y), wmin(z, w)), wmax(wmax(x, y), wmax(z, w)));
+    return I(wmin4(x, y, z, w), wmax4(x, y, z, w));
 }
 TSI_DI Iv MUL_IF(Iv a, float b) {                                                        // 2563
     const float x = a.lo * b, y = a.hi * b;
@@ -200,7 +202,7 @@ TSI_DI Iv EXP_I(Iv a) { return I(det_exp(a.lo), det_exp(a.hi)); }               
 TSI_DI Iv LOG_I(Iv a) { return I(det_log(wmax(a.lo, 0.0f)), det_log(wmax(a.hi, 0.0f))); }  // 2584
 TSI_DI Iv DIV_II(Iv a, Iv b) {                                                           // 2598
     float x = a.lo / b.lo, y = a.lo / b.hi, z = a.hi / b.lo, w = a.hi / b.hi;
-    return I(wmin(wmin(x, y), wmin(z, w)), wmax(wmax(x, y), wmax(z, w)));
+    return I(wmin4(x, y, z, w), wmax4(x, y, z, w));
 }
 TSI_DI Iv DIV_IF(Iv a, float b) {                                                        // 2609 (= DIV_II(a, [b,b]))
     float x = a.lo / b, z = a.hi / b;

@@ -124,6 +124,16 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     o += "static constexpr uint32_t kRegionWords = " + std::to_string(RW) + "u, kModeWords = " + std::to_string(MW) +
          "u;\n";
     o += "struct RegionMask { uint32_t w[" + std::to_string(RW) + "]; };\n";
+    // fix() with its bounds in scalar registers: a 4-byte encoding per use instead of an
+    // 8-byte one with a literal (the tape functions' code is streamed from L2 by every wave)
+    o += "struct FixK { float lo, hi; };\n"
+         "TSI_DI FixK fix_consts() {\n    uint32_t lo, hi;\n"
+         "    asm volatile(\"s_mov_b32 %0, 0xc9742400\" : \"=s\"(lo));\n"
+         "    asm volatile(\"s_mov_b32 %0, 0x49742400\" : \"=s\"(hi));\n"
+         "    return FixK{__uint_as_float(lo), __uint_as_float(hi)};\n}\n"
+         "TSI_DI Iv fixk(Iv x, FixK k) {   // constant operands keep fix() so that it folds\n"
+         "    if (__builtin_constant_p(x.lo) && __builtin_constant_p(x.hi)) return fix(x);\n"
+         "    return I(wmax(x.lo, k.lo), wmin(x.hi, k.hi));\n}\n";
     o += "struct FwdOut { Iv field; uint32_t cost, records; };\n";
     o += "struct RevOut { Iv s0, s1, s2, s3; uint32_t ops; };\n\n";
 
@@ -147,7 +157,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // registers of either alone, with spills); a call costs a few instructions per wavefront.
     o += "template <bool PUSH>\n__device__ __noinline__ FwdOut tape_forward(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
          "uint32_t lane, const uint32_t *prec, uint32_t cap, uint32_t *chbuf, RegionMask mask) {\n";
-    o += "    const Iv ZI = I_FROM(0.0f);\n    uint32_t cost = 0, recs = 0;\n";
+    o += "    const Iv ZI = I_FROM(0.0f);\n    const FixK K = fix_consts();\n    uint32_t cost = 0, recs = 0;\n";
     o += prologue(true);
     if (prune)
         for (uint32_t w = 0; w < MW; ++w) o += "    uint32_t cw" + std::to_string(w) + " = 0xFFFFFFFFu;\n";
@@ -174,15 +184,15 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
             const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
             o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
-            o += "        " + V(j) + " = fix(sel(md" + J + " == 1u, " + l + ", sel(md" + J + " == 2u, " + rhs +
-                 ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "))));\n";
+            o += "        " + V(j) + " = fixk(sel(md" + J + " == 1u, " + l + ", sel(md" + J + " == 2u, " + rhs +
+                 ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "))), K);\n";
             const int m = mmx[j];
             const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
             o += "        { const uint32_t ch = md" + J + " != 3u ? md" + J + " : " +
                  (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ");\n";
             o += "          cw" + w + " = (cw" + w + " & ~(3u << " + sh + ")) | (ch << " + sh + "); }\n";
         } else {
-            o += "        " + V(j) + " = fix(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "));\n";
+            o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
         }
         const uint32_t npush = f.npush_side & 3u;
         if (npush) {
@@ -209,7 +219,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     if (rev_kernel) {
         o += "__device__ __noinline__ RevOut tape_reverse(const float2 *stk, uint32_t lane, const uint32_t *prec, "
              "uint32_t cap, RegionMask mask) {\n";
-        o += "    const Iv ZI = I_FROM(0.0f), ONE = I_FROM(1.0f);\n    uint32_t ops = 0;\n";
+        o += "    const Iv ZI = I_FROM(0.0f), ONE = I_FROM(1.0f);\n    const FixK K = fix_consts();\n    uint32_t ops = 0;\n";
         o += prologue(true);
         std::vector<char> touched(n, 0), live(n, 0);
         for (uint32_t j = 4; j < n; ++j) {
@@ -284,9 +294,20 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                              lit(g.literal) + ");\n";
                     }
                     const std::string cost = std::to_string(unsigned(rv.cost)) + "u";
+                    // MUL_RR's derivatives are the saved operand values; a record's value
+                    // already went through fix, which is idempotent, so it is not applied again
+                    // (arguments are not fixed in the forward pass and keep it)
+                    auto fixed_d = [&](const char *dx) {
+                        const std::string d = "d" + J + "." + dx;
+                        if (f.form == F_MUL_RR && (dx[0] == 'a' ? b : a) >= 4u) return d;
+                        return "fixk(" + d + ", K)";
+                    };
+                    // the first contribution to a sensitivity is not added to [+0, +0]: that
+                    // addition only turns a -0 into +0, and no output depends on the sign of a
+                    // zero sensitivity (dx/dy/dz are stored with +0, the time error takes |df|)
                     auto contrib = [&](uint32_t k, const char *dx) {
-                        return "ADD_II(MUL_II(fix(d" + J + "." + dx + "), " + sj + "), " +
-                               (live[k] ? S(k) : std::string("ZI")) + ")";
+                        const std::string prod = "MUL_II(" + fixed_d(dx) + ", " + sj + ")";
+                        return live[k] ? "ADD_II(" + prod + ", " + S(k) + ")" : prod;
                     };
                     auto old = [&](uint32_t k) { return live[k] ? S(k) : std::string("ZI"); };
                     if (side == 1 || side == 2) {
@@ -337,7 +358,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         o += "ops};\n}\n\n";
     }
 
-    // ---- liveness (row P): the records reachable from the output through this cell's own
+    // ---- liveness: the records reachable from the output through this cell's own
     // MIN/MAX choices (compute.wgsl:2884-2903), as region bits and per-MIN/MAX modes for the
     // children's evaluation.  Regions skipped for the parent cannot hold live records.
     if (prune) {
