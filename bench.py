@@ -167,25 +167,48 @@ def main():
     frame = 0
 
     def step():
-        # the trace of frame k runs on a second stream while frame k+1's set inversion runs
-        nonlocal frame
+        """Queues frame k (set inversion, trace, root re-enqueue) and returns frame k-1's result:
+        the host never waits for the frame it has just queued, so the device always has the
+        next frame's work; the trace of frame k runs on a second stream beside the set
+        inversion of frame k+1."""
+        nonlocal frame, inflight
         st.update(float(np.float32(frame / 60.0)))
-        r = st.set_inversion()
-        if args.sync_trace:
-            st.render(pixels=False)
-        else:
-            st.render_async()
         frame += 1
-        return r
+        if args.sync_trace:
+            r = st.set_inversion()
+            st.render(pixels=False)
+            return [r]
+        st.set_inversion_async()
+        st.render_async()
+        inflight += 1
+        if inflight < 2:
+            return []
+        inflight -= 1
+        return [st.set_inversion_result()]
 
+    def drain():
+        nonlocal inflight
+        out = []
+        while inflight:
+            out.append(st.set_inversion_result())
+            inflight -= 1
+        return out
+
+    inflight = 0
     for _ in range(args.warmup):
         step()
+    drain()
     st.trace_wait(pixels=False)
+    frame_results = []
     ms0, visits0, n0 = st.trace_totals()
     tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0, pruned=0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r = step()
+        frame_results += step()
+    frame_results += drain()
+    st.trace_wait(pixels=False)   # the last frame's trace is part of the timed work
+    wall = time.perf_counter() - t0
+    for r in frame_results:
         s = r.stats
         tot["cells"] += s["cell_evals"]
         tot["rev"] += s["rev_evals"]
@@ -195,8 +218,7 @@ def main():
         tot["eval_ms"] += s["eval_ms"]
         tot["launches"] += s["eval_launches"]
         tot["pruned"] += s["pruned_instr"]
-    st.trace_wait(pixels=False)   # the last frame's trace is part of the timed work
-    wall = time.perf_counter() - t0
+    assert len(frame_results) == args.steps
     ms1, visits1, n1 = st.trace_totals()
     assert n1 - n0 == args.steps
     trace = dict(ms=ms1 - ms0, visits=visits1 - visits0)
@@ -228,7 +250,7 @@ def main():
                    "lipschitz_corrected": w["lipschitz_corrected"], "swept_volume": w["swept_volume"],
                    "point_sample": w["point_sample"], "times": "t_k = k/60", "engine": info["engine"],
                    "trace": "synchronous" if args.sync_trace else "overlapped with the next set inversion",
-                   "prune": args.prune if info["engine"] == "interpreter" else "n/a"},
+                   "prune": args.prune, "frames_in_flight": 1 if args.sync_trace else 2},
         "tape_compile_s": info["compile_ms"] / 1e3, "tape_compile_cached": bool(info["cache_hit"]),
         "records_run_per_cell": (len(tape) - 4) - tot["pruned"] / max(tot["cells"], 1),
         "set_inversion_ms_per_frame": tot["gpu_ms"] / args.steps,

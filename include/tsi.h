@@ -30,7 +30,8 @@ enum {
     TSI_E_NO_TAPE = -8,          /* no tape uploaded yet */
     TSI_E_UNSUPPORTED = -9,      /* tape needs more live values than the register file holds */
     TSI_E_BUDGET = -10,          /* served form: request exceeds the HBM admission budget */
-    TSI_E_JIT = -11              /* the tape's evaluation kernels did not compile (see tsi_tape_info) */
+    TSI_E_JIT = -11,             /* the tape's evaluation kernels did not compile (see tsi_tape_info) */
+    TSI_E_PENDING = -12          /* frames in flight: collect a tsi_set_inversion_result first */
 };
 
 /* src/ir.rs:88-97 GPUInstruction, 32 bytes: opcode_uses = (op | side<<14) << 16 | uses,
@@ -69,6 +70,10 @@ typedef struct {
     uint32_t flags;               /* TSI_FLAG_* */
     uint32_t prune_capacity;      /* frontier entries per list and level that get a pruned
                                      program (0 = min(queue_capacity, 2^19)) */
+    uint32_t prune_live_max;      /* compiled engine: a wavefront runs its pruned code when at
+                                     most this percentage of the tape's regions is live for it
+                                     (0 = 75; 100 = whenever its parents' records apply) */
+    uint32_t reserved;
 } tsi_params;
 
 enum {
@@ -80,8 +85,8 @@ enum {
     TSI_FLAG_NO_PRUNE = 1u,
     TSI_FLAG_PRUNE_EXACT_ONLY = 2u, /* prune only tapes of the first (exact) kind */
     /* Evaluate tapes with the interpreter (programs in constant memory) instead of compiling
-     * each uploaded tape into straight-line evaluation kernels.  Pruning applies to the
-     * interpreter only: compiled kernels always run the full tape. */
+     * each uploaded tape into straight-line evaluation kernels (which prune by regions of
+     * 16 records, see prune_live_max). */
     TSI_FLAG_INTERPRET = 4u
 };
 
@@ -135,6 +140,15 @@ int tsi_set_uniforms(tsi_ctx *ctx, const tsi_uniforms *u);
  * Blocking.  *gpu_ms (may be NULL) receives the device time, like State::compute's return. */
 int tsi_set_inversion(tsi_ctx *ctx, double *gpu_ms, tsi_stats *stats);
 
+/* The same frame without waiting for it (no counterpart in the reference, whose frame loop
+ * waits on every dispatch): the frame is queued behind the previous work of the context and
+ * its counters are copied to pinned host memory in stream order.  At most two frames may be in
+ * flight (TSI_E_PENDING otherwise); tsi_set_inversion_result waits for the oldest one and
+ * returns what tsi_set_inversion would have (its return code is that frame's).  While frames are
+ * in flight, tsi_set_inversion, tsi_upload_tape and tsi_reset return TSI_E_PENDING. */
+int tsi_set_inversion_async(tsi_ctx *ctx);
+int tsi_set_inversion_result(tsi_ctx *ctx, double *gpu_ms, tsi_stats *stats);
+
 /* rt_main (src/main.rs:649-656, compute.wgsl:4650-4688): traces the frame into the
  * library's device pixel buffer and copies it to rgba_out (W*H*4 floats, row 0 on top)
  * and depth_out (W*H floats, -1 = miss); either may be NULL. */
@@ -142,9 +156,11 @@ int tsi_trace(tsi_ctx *ctx, float *rgba_out, float *depth_out);
 
 /* The same trace without waiting (no counterpart in the reference, whose render call is
  * synchronous): the nodes in use are copied to a device-side snapshot in stream order after
- * the last set inversion, and a second stream traces the snapshot while the caller goes on
- * with the next frame's tsi_set_inversion.  tsi_trace_wait waits for it and copies the frame
- * out (rgba / depth may be NULL).  A new tsi_trace_async first waits for the previous one. */
+ * the last set inversion (the count is read on the device: no host wait), and a second stream
+ * traces the snapshot while the caller goes on with the next frame's set inversion.  The
+ * snapshot holds 48 bytes per node of voxel_capacity, allocated at the first asynchronous trace.
+ * tsi_trace_wait waits for every trace in flight and copies the last frame out (rgba / depth may
+ * be NULL).  The snapshot of a new tsi_trace_async waits on the device for the previous trace. */
 int tsi_trace_async(tsi_ctx *ctx);
 int tsi_trace_wait(tsi_ctx *ctx, float *rgba_out, float *depth_out);
 /* Device time and node visits summed over the traces completed since the context was made. */

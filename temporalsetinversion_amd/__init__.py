@@ -153,7 +153,7 @@ class State:
                  point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
                  fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
                  voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0,
-                 engine: str = "compiled"):
+                 engine: str = "compiled", prune_live_max: int = 0):
         if isinstance(tape, IR):
             tape = tape.gpu_instructions
         elif isinstance(tape, (str, bytes)):
@@ -175,13 +175,15 @@ class State:
             p.flags = _lib.TSI_FLAG_PRUNE_EXACT_ONLY
         else:
             p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
-        # engine: "compiled" = the tape compiled into straight-line kernels at upload (full tape
-        # per cell), "interpreter" = programs in constant memory, pruned per frontier group
+        # engine: "compiled" = the tape compiled into straight-line kernels at upload (pruned by
+        # regions of 16 records), "interpreter" = programs in constant memory, pruned per
+        # frontier group and record
         if engine not in ("compiled", "interpreter"):
             raise ValueError(f"engine must be 'compiled' or 'interpreter', not {engine!r}")
         if engine == "interpreter":
             p.flags |= _lib.TSI_FLAG_INTERPRET
         p.prune_capacity = prune_capacity
+        p.prune_live_max = prune_live_max
         self.params = p
         self._ctx = C.c_void_p()
         check(lib().tsi_create(C.byref(p), C.byref(self._ctx)), "tsi_create")
@@ -227,6 +229,17 @@ class State:
         ms = C.c_double(0.0)
         st = Stats()
         check(lib().tsi_set_inversion(self._ctx, C.byref(ms), C.byref(st)), "tsi_set_inversion")
+        return FrameResult(ms.value, st.as_dict())
+
+    def set_inversion_async(self):
+        """Queues the frame without waiting for it (at most two in flight)."""
+        check(lib().tsi_set_inversion_async(self._ctx), "tsi_set_inversion_async")
+
+    def set_inversion_result(self) -> FrameResult:
+        """Waits for the oldest frame in flight; its statistics, or its error raised."""
+        ms = C.c_double(0.0)
+        st = Stats()
+        check(lib().tsi_set_inversion_result(self._ctx, C.byref(ms), C.byref(st)), "tsi_set_inversion_result")
         return FrameResult(ms.value, st.as_dict())
 
     def trace(self, pixels: bool = True):

@@ -19,7 +19,7 @@ TSI_OK = 0
 ERRORS = {
     -1: "TSI_E_OOM", -2: "TSI_E_QUEUE_FULL", -3: "TSI_E_POOL_EXHAUSTED", -4: "TSI_E_TAPE_TOO_LONG",
     -5: "TSI_E_HIP", -6: "TSI_E_ARG", -7: "TSI_E_PARSE", -8: "TSI_E_NO_TAPE", -9: "TSI_E_UNSUPPORTED",
-    -10: "TSI_E_BUDGET", -11: "TSI_E_JIT",
+    -10: "TSI_E_BUDGET", -11: "TSI_E_JIT", -12: "TSI_E_PENDING",
 }
 TSI_FLAG_NO_PRUNE = 1
 TSI_FLAG_PRUNE_EXACT_ONLY = 2
@@ -49,7 +49,8 @@ class Params(C.Structure):
     _fields_ = [("localsubdiv", C.c_int32), ("delta", C.c_float), ("point_sample", C.c_int32),
                 ("swept_volume", C.c_int32), ("lipschitz_corrected", C.c_int32), ("fastenabled", C.c_int32),
                 ("fast_multiplier", C.c_float), ("swept_window", C.c_float), ("voxel_capacity", C.c_uint64),
-                ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("prune_capacity", C.c_uint32)]
+                ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("prune_capacity", C.c_uint32),
+                ("prune_live_max", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -84,6 +85,8 @@ SIGNATURES = {
     "tsi_reset": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "tsi_set_uniforms": (C.c_int, [C.c_void_p, C.POINTER(Uniforms)]),
     "tsi_set_inversion": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(Stats)]),
+    "tsi_set_inversion_async": (C.c_int, [C.c_void_p]),
+    "tsi_set_inversion_result": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(Stats)]),
     "tsi_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tsi_trace_async": (C.c_int, [C.c_void_p]),
     "tsi_trace_wait": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -6,15 +6,18 @@
 //
 // Engine (evaluation of the tape for the wavefront's 64 cells):
 //   Engine(a, wave, lane)
-//   select(a, which, g, pruned_ok, prec_in, rec_idx)  program for group g (pruned_ok: every
-//                                   parent has a liveness record and the group's cells lie inside
-//                                   their parents; the lane's parent record is prec_in + rec_idx)
+//   select(a, which, g, pruned_ok, prec_in, rec_idx, need_choices)  program for group g
+//                                   (pruned_ok: every parent has a liveness record and the
+//                                   group's cells lie inside their parents; the lane's parent
+//                                   record is prec_in + rec_idx; need_choices: some cell may
+//                                   write a liveness record its children can use)
 //   forward<PUSH>(a, x, y, z, t, stk, field, Rlo, Rhi) -> algorithmic f32 ops; PUSH saves the
 //                                   operands the reverse rules need on the wavefront's stack
 //                                   (stk: uniform base, row r of lane l at stk[64 r + l])
 //   reverse(a, stk, sens[4], Rlo, Rhi) -> algorithmic f32 ops; sensitivities of x, y, z, t
 //   (Rlo/Rhi: a register file the engine may use between the two calls)
 //   records()                      records run per forward pass
+//   choices_recorded()             the forward pass recorded the MIN/MAX choices
 //   liveness(a, rec)               the cell's liveness record (kPrune engines)
 #pragma once
 #include "tsi_wave.h"
@@ -140,7 +143,24 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
                                 cy.lo >= plo.y && cy.hi <= phi.y && cz.lo >= plo.z && cz.hi <= phi.z;
             pruned_ok = ballot(guard && !inside) == 0;
         }
-        E.select(a, which, g, pruned_ok, prec_in, rec_idx);
+        // A liveness record of this cell serves its children only if their t-windows will lie
+        // inside this cell's: children that do not exist yet, or whose windows will be
+        // degenerate, start at `time`; existing children start at their own last evaluation,
+        // which must not precede this cell's (it does when this cell was evaluated without
+        // being subdivided since).  A cell later found fast also gives degenerate windows, but
+        // that is not known yet.
+        bool kids_ok = false;
+        if (prune) {
+            kids_ok = a.swept || (cmeta & kChildMask) == 0 || width / 2.0f <= eps;
+            if (guard && !kids_ok) {
+                const uint32_t cb = ((cmeta & kChildMask) - 1u) * 8u;
+                float tmin = a.st[cb].y;
+                for (uint32_t k = 1; k < 8; ++k) tmin = fminf(tmin, a.st[cb + k].y);
+                kids_ok = tmin >= targ.lo;
+            }
+        }
+        const bool need_choices = prune && ballot(guard && kids_ok && leaf_w > eps) != 0;
+        E.select(a, which, g, pruned_ok, prec_in, rec_idx, need_choices);
 
         // ---- tape evaluation (wave-uniform) ----
         Iv field = I_FROM(0.0f);
@@ -226,15 +246,18 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
                     &cnt->frame_error, kErrQueue);
 
         // ---- liveness of the enqueued cells, for their children's pruned programs ----
+        // (a record the children cannot use gets an empty window: no t-window lies inside it)
         if (prune && ballot(enq) != 0) {
             uint32_t *rec = nullptr;
             if (at_f < a.prune_cap) rec = prec_f + at_f;
             if (at_r < a.prune_cap) rec = prec_r + at_r;
+            const bool valid = kids_ok && E.choices_recorded();
             if (rec) {
-                rec[0] = __float_as_uint(targ.lo);
-                rec[a.prune_cap] = __float_as_uint(targ.hi);
+                rec[0] = valid ? __float_as_uint(targ.lo) : 0x7f800000u;
+                rec[a.prune_cap] = valid ? __float_as_uint(targ.hi) : 0xff800000u;
             }
-            E.liveness(a, rec);
+            if (!valid) rec = nullptr;
+            if (ballot(rec != nullptr) != 0) E.liveness(a, rec);
         }
         n_cells += guard ? 1 : 0;
         n_rev += (guard && td) ? 1 : 0;

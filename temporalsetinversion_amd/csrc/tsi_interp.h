@@ -359,7 +359,7 @@ struct InterpEngine {
         ch.n = 0u;
     }
 
-    TSI_DI void select(const KArgs &a, int which, uint32_t g, bool pruned_ok, const uint32_t *, uint32_t) {
+    TSI_DI void select(const KArgs &a, int which, uint32_t g, bool pruned_ok, const uint32_t *, uint32_t, bool) {
         fprog = a.fprog;
         rprog = a.rprog;
         n = a.nfwd;
@@ -396,6 +396,7 @@ struct InterpEngine {
     }
 
     TSI_DI uint32_t records() const { return n; }
+    TSI_DI bool choices_recorded() const { return choices; }
 
     TSI_DI void liveness(const KArgs &a, uint32_t *rec) { live_walk(fprog, n, a.out_slot, ch, a.nchunks, rec, a.prune_cap); }
 };

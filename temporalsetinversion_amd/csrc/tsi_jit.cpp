@@ -103,16 +103,18 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     std::vector<int> mmx(n, -1);
     for (uint32_t j = 4, m = 0; j < n; ++j)
         if (is_minmax_form(fwd[j - 4].form)) mmx[j] = int(m++);
+    // the variant being emitted: pruned (region mask, parent modes, choices) or full
+    bool pr = prune;
     auto bit = [&](uint32_t r) {   // uniform test of region r's bit in the wave's region mask
-        return prune ? "((rm" + std::to_string(r / 32) + " >> " + std::to_string(r % 32) + ") & 1u)"
+        return pr ? "((rm" + std::to_string(r / 32) + " >> " + std::to_string(r % 32) + ") & 1u)"
                      : std::string("go");
     };
     auto open_region = [&](uint32_t r) {
-        if (prune) return "    if (" + bit(r) + ") {\n";
+        if (pr) return "    if (" + bit(r) + ") {\n";
         return std::string("    {\n        uint32_t go;\n        asm volatile(\"s_mov_b32 %0, 1\" : \"=s\"(go));\n"
                            "        if (go) {\n");
     };
-    auto close_region = [&]() { return prune ? std::string("    }\n") : std::string("        }\n    }\n"); };
+    auto close_region = [&]() { return pr ? std::string("    }\n") : std::string("        }\n    }\n"); };
     auto mode_of = [&](uint32_t j) {
         const int m = mmx[j];
         return "((mw" + std::to_string(m / 16) + " >> " + std::to_string(2 * (m % 16)) + "u) & 3u)";
@@ -140,7 +142,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // common prologue of the pruned functions: uniform region words, this lane's parent modes
     auto prologue = [&](bool need_modes) {
         std::string p;
-        if (!prune) return p;
+        if (!pr) return p;
         for (uint32_t w = 0; w < RW; ++w)
             p += "    const uint32_t rm" + std::to_string(w) + " = __builtin_amdgcn_readfirstlane(mask.w[" +
                  std::to_string(w) + "]);\n";
@@ -155,13 +157,15 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // The tape functions are calls, not inlined into the level kernel: inlined, the register
     // allocation of the kernel's own state and of the tape interfere (several times the
     // registers of either alone, with spills); a call costs a few instructions per wavefront.
-    o += "template <bool PUSH>\n__device__ __noinline__ FwdOut tape_forward(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
+    // choices: the MIN/MAX choices go to chbuf (always with pr)
+    auto emit_forward = [&](const std::string &fname, bool choices) {
+    o += "template <bool PUSH>\n__device__ __noinline__ FwdOut " + fname + "(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
          "uint32_t lane, const uint32_t *prec, uint32_t cap, uint32_t *chbuf, RegionMask mask) {\n";
     o += "    const Iv ZI = I_FROM(0.0f);\n    const FixK K = fix_consts();\n    uint32_t cost = 0, recs = 0;\n";
     o += prologue(true);
-    if (prune)
-        for (uint32_t w = 0; w < MW; ++w) o += "    uint32_t cw" + std::to_string(w) + " = 0xFFFFFFFFu;\n";
-    for (uint32_t j = 4; j < n; ++j) o += "    Iv " + V(j) + (prune ? " = ZI;\n" : ";\n");
+    if (choices)   // skipped regions leave their MIN/MAX undecided (3)
+        for (uint32_t w = 0; w < MW; ++w) o += "    uint32_t cw" + std::to_string(w) + (pr ? " = 0xFFFFFFFFu;\n" : " = 0u;\n");
+    for (uint32_t j = 4; j < n; ++j) o += "    Iv " + V(j) + (pr ? " = ZI;\n" : ";\n");
     for (uint32_t j = 4; j < n; ++j) {
         const auto &g = tape[j];
         const FwdIns &f = fwd[j - 4];
@@ -177,7 +181,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         const std::string rr = (side == 0 || side == 2) ? V(rarg_of(g)) : "ZI";
         const std::string c = lit(g.literal);
         const std::string form = std::to_string(f.form) + "u";
-        if (prune && mmx[j] >= 0) {
+        if (pr && mmx[j] >= 0) {
             // the parent's decided choice makes the MIN/MAX a copy (tsi_interp.h fwd_step)
             const std::string J = std::to_string(j);
             const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
@@ -191,6 +195,14 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             o += "        { const uint32_t ch = md" + J + " != 3u ? md" + J + " : " +
                  (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ");\n";
             o += "          cw" + w + " = (cw" + w + " & ~(3u << " + sh + ")) | (ch << " + sh + "); }\n";
+        } else if (choices && mmx[j] >= 0) {
+            const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
+            const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
+            const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
+            o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
+            const int m = mmx[j];
+            const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
+            o += "        cw" + w + " |= " + (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ") << " + sh + ";\n";
         } else {
             o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
         }
@@ -207,17 +219,18 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         }
         if ((j - 4) % G == G - 1 || j == n - 1) o += close_region();
     }
-    if (prune)
+    if (choices)
         for (uint32_t w = 0; w < MW; ++w)
             o += "    chbuf[" + std::to_string(w * 64u) + "u] = cw" + std::to_string(w) + ";\n";
     o += "    return FwdOut{" + V(n - 1) + ", cost, recs};\n}\n\n";
+    };
 
     // ---- reverse (compute.wgsl:3780-3964) ----
     // sens of instruction k is s<k>, [+0, +0] until its first contribution (j descending).
     // Temporaries of record j carry the suffix _j.  Decided MIN/MAX pass the contribution to
     // the kept operand only (tsi_interp.h rev_step).
-    if (rev_kernel) {
-        o += "__device__ __noinline__ RevOut tape_reverse(const float2 *stk, uint32_t lane, const uint32_t *prec, "
+    auto emit_reverse = [&](const std::string &fname) {
+        o += "__device__ __noinline__ RevOut " + fname + "(const float2 *stk, uint32_t lane, const uint32_t *prec, "
              "uint32_t cap, RegionMask mask) {\n";
         o += "    const Iv ZI = I_FROM(0.0f), ONE = I_FROM(1.0f);\n    const FixK K = fix_consts();\n    uint32_t ops = 0;\n";
         o += prologue(true);
@@ -232,7 +245,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (side != 3 && npush == 2) o += "    float2 q_" + std::to_string(j) + ";\n";
         }
         for (uint32_t k = 0; k < n; ++k)
-            if (touched[k]) o += "    Iv " + S(k) + (prune ? " = ZI;\n" : ";\n");
+            if (touched[k]) o += "    Iv " + S(k) + (pr ? " = ZI;\n" : ";\n");
         uint64_t static_ops = 0;
         auto emit_loads = [&](uint32_t hi, uint32_t lo) {   // saved operands of records hi-1 .. lo
             for (uint32_t jj = hi; jj-- > lo;) {
@@ -246,7 +259,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         };
         // regions from the top: records [hi, lo) with lo the region start
         auto region_lo = [&](uint32_t hi) { return 4u + ((hi - 5u) / G) * G; };
-        if (!prune) {
+        if (!pr) {
             o += open_region(0);
             emit_loads(n, region_lo(n));
             o += close_region();
@@ -254,7 +267,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) {
             const uint32_t lo = region_lo(hi), r = region_of(lo);
             o += open_region(r);
-            if (prune) {
+            if (pr) {
                 // operands of this region: loaded here unless the region above (run before it)
                 // prefetched them
                 if (hi == n) emit_loads(hi, lo);
@@ -281,7 +294,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     const std::string x = npush ? "I(p" + J + ".x, p" + J + ".y)" : "ZI";
                     const std::string y = npush == 2 ? "I(q" + J + ".x, q" + J + ".y)" : "ZI";
                     const std::string form = std::to_string(f.form) + "u";
-                    const bool mm = prune && mmx[j] >= 0;
+                    const bool mm = pr && mmx[j] >= 0;
                     if (mm) {
                         o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
                         o += "        Iv2 d" + J + " = rev_form_t<true>(" + form + ", " + x + ", " + y + ", " +
@@ -356,6 +369,19 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         o += "    return RevOut{";
         for (uint32_t k = 0; k < 4; ++k) o += (live[k] ? S(k) : std::string("ZI")) + ", ";
         o += "ops};\n}\n\n";
+    };
+
+    // pruned modules carry three forward variants: pruned (the parents' records apply and skip
+    // enough regions), full with the MIN/MAX choices recorded (some cell may write a liveness
+    // record its children can use) and full; and two reverse ones, pruned and full
+    emit_forward("tape_forward", prune);
+    if (rev_kernel) emit_reverse("tape_reverse");
+    if (prune) {
+        pr = false;
+        emit_forward("tape_forward_full", false);
+        emit_forward("tape_forward_choices", true);
+        if (rev_kernel) emit_reverse("tape_reverse_full");
+        pr = true;
     }
 
     // ---- liveness: the records reachable from the output through this cell's own
@@ -421,15 +447,23 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
 
     // ---- the evaluation engine of eval_level (tsi_eval.h) ----
     o += std::string("struct TapeEngine {\n    static constexpr bool kPrune = ") + (prune ? "true" : "false") + ";\n";
-    o += "    uint32_t lane, recs;\n    const uint32_t *prec;\n    uint32_t *chbuf;\n    RegionMask mask;\n";
-    o += "    TSI_DI TapeEngine(const KArgs &a, uint32_t wave, uint32_t l) : lane(l), recs(kTapeRecords), prec(nullptr) {\n"
+    o += "    static constexpr uint32_t kRegions = " + std::to_string(Y.regions) + "u;\n";
+    o += "    uint32_t lane, recs;\n    const uint32_t *prec;\n    uint32_t *chbuf;\n    RegionMask mask;\n"
+         "    uint32_t variant;   // this wavefront's: 0 full, 1 full + choices, 2 pruned\n";
+    o += "    TSI_DI TapeEngine(const KArgs &a, uint32_t wave, uint32_t l)\n"
+         "        : lane(l), recs(kTapeRecords), prec(nullptr), variant(0) {\n"
          "        for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n"
          "        chbuf = a.choice ? a.choice + size_t(wave) * kModeWords * 64u + l : nullptr;\n    }\n";
-    o += "    TSI_DI void select(const KArgs &a, int, uint32_t, bool pruned_ok, const uint32_t *prec_in, uint32_t pidx) {\n"
+    o += "    TSI_DI void select(const KArgs &a, int, uint32_t, bool pruned_ok, const uint32_t *prec_in, uint32_t pidx,\n"
+         "                       bool need_choices) {\n"
          "        prec = nullptr;\n"
+         "        variant = 0;\n"
          "        for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n";
     if (prune)
-        o += "        if (pruned_ok) {\n"
+        // the pruned variant costs about a third more than the full one per region it runs:
+        // it is taken when at most prune_live_pct % of the regions are live (default 75)
+        o += "        uint32_t live = 0;\n"
+             "        if (pruned_ok) {\n"
              "            prec = prec_in + pidx;\n"
              "            for (uint32_t w = 0; w < kRegionWords; ++w) {\n"
              "                uint32_t v = prec[size_t(2u + w) * a.prune_cap];\n"
@@ -437,20 +471,37 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
              "                v |= uint32_t(__shfl_xor(int(v), 16, 64));\n"
              "                v |= uint32_t(__shfl_xor(int(v), 32, 64));\n"
              "                mask.w[w] = v;\n"
-             "            }\n        }\n";
+             "                live += uint32_t(__popc(__builtin_amdgcn_readfirstlane(v)));\n"
+             "            }\n        }\n"
+             "        variant = pruned_ok && live * 100u <= kRegions * a.prune_live_pct ? 2u : (need_choices ? 1u : 0u);\n"
+             "        if (variant != 2u) {\n"
+             "            prec = nullptr;\n"
+             "            for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n"
+             "        }\n";
     o += "    }\n";
+    const std::string fcall = "(x, y, z, t, stk, lane, prec, a.prune_cap, chbuf, mask)";
     o += "    template <bool PUSH>\n"
-         "    TSI_DI uint32_t forward(const KArgs &a, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n"
-         "        const FwdOut r = tape_forward<PUSH>(x, y, z, t, stk, lane, prec, a.prune_cap, chbuf, mask);\n"
-         "        field = r.field;\n        recs = r.records;\n        return r.cost;\n    }\n";
-    if (rev_kernel)
-        o += "    TSI_DI uint32_t reverse(const KArgs &a, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n"
-             "        const RevOut r = tape_reverse(stk, lane, prec, a.prune_cap, mask);\n"
-             "        sens[0] = r.s0; sens[1] = r.s1; sens[2] = r.s2; sens[3] = r.s3;\n        return r.ops;\n    }\n";
+         "    TSI_DI uint32_t forward(const KArgs &a, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n";
+    if (prune)
+        o += "        const FwdOut r = variant == 2u ? tape_forward<PUSH>" + fcall + "\n"
+             "                       : variant == 1u ? tape_forward_choices<PUSH>" + fcall + "\n"
+             "                                       : tape_forward_full<PUSH>" + fcall + ";\n";
     else
+        o += "        const FwdOut r = tape_forward<PUSH>" + fcall + ";\n";
+    o += "        field = r.field;\n        recs = r.records;\n        return r.cost;\n    }\n";
+    const std::string rcall = "(stk, lane, prec, a.prune_cap, mask)";
+    if (rev_kernel) {
+        o += "    TSI_DI uint32_t reverse(const KArgs &a, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n";
+        if (prune)
+            o += "        const RevOut r = variant == 2u ? tape_reverse" + rcall + " : tape_reverse_full" + rcall + ";\n";
+        else o += "        const RevOut r = tape_reverse" + rcall + ";\n";
+        o += "        sens[0] = r.s0; sens[1] = r.s1; sens[2] = r.s2; sens[3] = r.s3;\n        return r.ops;\n    }\n";
+    } else {
         o += "    TSI_DI uint32_t reverse(const KArgs &, const float2 *, Iv sens[4], Slots &, Slots &) {\n"
              "        for (int k = 0; k < 4; ++k) sens[k] = I_FROM(0.0f);\n        return 0u;\n    }\n";
+    }
     o += "    TSI_DI uint32_t records() const { return recs; }\n";
+    o += "    TSI_DI bool choices_recorded() const { return variant != 0u; }\n";
     if (prune)
         o += "    TSI_DI void liveness(const KArgs &a, uint32_t *rec) { tape_liveness(chbuf, rec, a.prune_cap, mask); }\n";
     else

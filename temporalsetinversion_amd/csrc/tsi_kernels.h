@@ -66,6 +66,7 @@ struct KArgs {
     uint32_t prune_cap;         // entries per list that get a record
     uint32_t nchunks;           // ceil(nfwd / 32)
     uint32_t *gdesc[2];         // per list: {pool offset, records} for each group of 8 entries
+    uint32_t prune_live_pct;    // compiled engine: pruned code at most this % of regions live
     uint32_t *choice;           // per wavefront: choice_len words x 64 lanes
     uint32_t choice_len;
     FwdIns *pool;               // program pool: forward records then reverse records per group
@@ -81,20 +82,5 @@ struct TraceArgs {
     unsigned long long *visits;
 };
 
-struct FrameLaunch {
-    int levels;
-    float time;
-    uint32_t grid_fwd, grid_rev, grid_misc;
-    void *tape_eval[2];   // compiled level kernels (hipFunction_t) of the tape, or null: k_eval
-};
-
-#ifndef __HIPCC_RTC__
-int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1);
-int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
-int launch_enqueue_root(const KArgs &a, hipStream_t s);
-int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
-int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s);
-
-#endif
 
 }  // namespace tsi

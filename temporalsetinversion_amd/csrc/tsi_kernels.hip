@@ -19,6 +19,7 @@
 
 #include "tsi_eval.h"
 #include "tsi_interp.h"
+#include "tsi_launch.h"
 
 namespace tsi {
 
@@ -455,6 +456,26 @@ int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t
 
 int launch_enqueue_root(const KArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enqueue_root, dim3(1), dim3(1), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Copy of the nodes in use for an asynchronous trace; the count is read on the device (the
+// host does not wait for the set inversion that produced it).
+__global__ void __launch_bounds__(256) k_snapshot(const Counters *cnt, const float4 *st, const float4 *blo,
+                                                  const float4 *bhi, float4 *s_st, float4 *s_blo, float4 *s_bhi,
+                                                  uint64_t nodes) {
+    uint64_t used = (uint64_t(cnt->bump) + 1u) * 8u;
+    if (used > nodes) used = nodes;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < used; i += uint64_t(gridDim.x) * blockDim.x) {
+        s_st[i] = st[i];
+        s_blo[i] = blo[i];
+        s_bhi[i] = bhi[i];
+    }
+}
+
+int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, uint64_t nodes, uint32_t grid,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, a.cnt, a.st, a.blo, a.bhi, s_st, s_blo, s_bhi, nodes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -14,7 +14,7 @@
 
 #include "tsi.h"
 #include "tsi_jit.h"
-#include "tsi_kernels.h"
+#include "tsi_launch.h"
 #include "tsi_tape.h"
 
 namespace tsi {
@@ -67,13 +67,25 @@ struct tsi_ctx {
     // asynchronous trace: a second stream traces a device-side snapshot of the octree while the
     // next frame's set inversion runs on the main stream
     hipStream_t tstream = nullptr;
-    hipEvent_t ev_snap = nullptr, ev_tdone = nullptr;
-    float4 *snap_st = nullptr, *snap_blo = nullptr, *snap_bhi = nullptr;
-    uint64_t snap_cap = 0;
-    uint64_t used_nodes = 8;            // nodes in use after the last set inversion
-    bool trace_pending = false;
+    hipEvent_t ev_snap = nullptr;
+    float4 *snap_st = nullptr, *snap_blo = nullptr, *snap_bhi = nullptr;   // voxel_capacity nodes
+    uint64_t used_nodes = 8;            // nodes in use after the last collected set inversion
     double trace_ms_total = 0.0;
     uint64_t trace_visits_total = 0, trace_count = 0;
+    // work in flight: frames (set inversions) and traces, each a ring of slots with their events
+    // and pinned host copies of what the host reads back
+    struct FrameSlot {
+        hipEvent_t begin = nullptr, end = nullptr, eval0 = nullptr, eval1 = nullptr;
+        Counters *h = nullptr;
+    };
+    struct TraceSlot {
+        hipEvent_t begin = nullptr, end = nullptr;
+        unsigned long long *visits_h = nullptr;
+    };
+    static constexpr uint32_t kFrameSlots = 2, kTraceSlots = 2;
+    FrameSlot fslot[kFrameSlots];
+    TraceSlot tslot[kTraceSlots];
+    uint64_t f_issued = 0, f_done = 0, t_issued = 0, t_done = 0;
 };
 
 namespace {
@@ -81,6 +93,7 @@ namespace {
 #define HIP_OK(x) do { if ((x) != hipSuccess) return TSI_E_HIP; } while (0)
 
 int harvest_trace(tsi_ctx *c);
+int harvest_all(tsi_ctx *c);
 
 template <class T>
 int dmalloc(T **p, size_t count) {
@@ -135,6 +148,7 @@ KArgs make_args(const tsi_ctx *c) {
     a.nchunks = c->nchunks;
     a.gdesc[0] = c->gdesc[0];
     a.gdesc[1] = c->gdesc[1];
+    a.prune_live_pct = c->p.prune_live_max ? (c->p.prune_live_max < 100u ? c->p.prune_live_max : 100u) : 75u;
     a.choice = c->choice;
     a.choice_len = c->choice_len;
     a.pool = c->pool;
@@ -248,12 +262,22 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
 }
 
 void release(tsi_ctx *c) {
+    if (c->stream) hipStreamSynchronize(c->stream);
     if (c->tstream) hipStreamSynchronize(c->tstream);
     release_jit(c);
     release_prune(c);
     dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
     if (c->ev_snap) { hipEventDestroy(c->ev_snap); c->ev_snap = nullptr; }
-    if (c->ev_tdone) { hipEventDestroy(c->ev_tdone); c->ev_tdone = nullptr; }
+    for (auto &f : c->fslot) {
+        for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1})
+            if (*e) { hipEventDestroy(*e); *e = nullptr; }
+        if (f.h) { hipHostFree(f.h); f.h = nullptr; }
+    }
+    for (auto &t : c->tslot) {
+        for (hipEvent_t *e : {&t.begin, &t.end})
+            if (*e) { hipEventDestroy(*e); *e = nullptr; }
+        if (t.visits_h) { hipHostFree(t.visits_h); t.visits_h = nullptr; }
+    }
     if (c->tstream) { hipStreamDestroy(c->tstream); c->tstream = nullptr; }
     dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
@@ -297,6 +321,7 @@ const char *tsi_strerror(int code) {
     case TSI_E_UNSUPPORTED: return "tape needs more live values than the register file holds";
     case TSI_E_BUDGET: return "request exceeds the admission budget";
     case TSI_E_JIT: return "the tape's evaluation kernels did not compile";
+    case TSI_E_PENDING: return "frames in flight: collect tsi_set_inversion_result first";
     default: return "unknown error";
     }
 }
@@ -322,13 +347,20 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreate(&c->ev_tdone) != hipSuccess) {
+            hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming) != hipSuccess) {
             rc = TSI_E_HIP;
             break;
         }
         bool evok = true;
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
+        for (auto &f : c->fslot) {
+            for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1}) evok = evok && hipEventCreate(e) == hipSuccess;
+            evok = evok && hipHostMalloc((void **)&f.h, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
+        }
+        for (auto &t : c->tslot) {
+            for (hipEvent_t *e : {&t.begin, &t.end}) evok = evok && hipEventCreate(e) == hipSuccess;
+            evok = evok && hipHostMalloc((void **)&t.visits_h, sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+        }
         if (!evok) { rc = TSI_E_HIP; break; }
         uint64_t nodes = params->voxel_capacity ? params->voxel_capacity : (uint64_t(1) << 24);
         nodes = (nodes + 7) / 8 * 8;
@@ -375,7 +407,9 @@ void tsi_destroy(tsi_ctx *c) {
 
 int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     if (!c || !tape) return TSI_E_ARG;
-    if (harvest_trace(c)) return TSI_E_HIP;
+    if (c->f_issued != c->f_done) return TSI_E_PENDING;
+    if (harvest_all(c)) return TSI_E_HIP;
+    HIP_OK(hipStreamSynchronize(c->stream));   // buffers below are replaced
     if (n > 4096) return TSI_E_TAPE_TOO_LONG;
     std::vector<FwdIns> fwd;
     std::vector<RevIns> rev;
@@ -430,7 +464,8 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
 
 int tsi_reset(tsi_ctx *c, const float root_box[6]) {
     if (!c || !root_box) return TSI_E_ARG;
-    if (harvest_trace(c)) return TSI_E_HIP;
+    if (c->f_issued != c->f_done) return TSI_E_PENDING;
+    if (harvest_all(c)) return TSI_E_HIP;
     KArgs a = make_args(c);
     if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -449,21 +484,32 @@ int tsi_enqueue_root(tsi_ctx *c) {
     return TSI_OK;
 }
 
-int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
+int tsi_set_inversion_async(tsi_ctx *c) {
     if (!c) return TSI_E_ARG;
     if (!c->have_tape) return TSI_E_NO_TAPE;
+    if (c->f_issued - c->f_done >= tsi_ctx::kFrameSlots) return TSI_E_PENDING;
+    tsi_ctx::FrameSlot &f = c->fslot[c->f_issued % tsi_ctx::kFrameSlots];
     KArgs a = make_args(c);
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
                    {(void *)c->eval_fn[0], (void *)c->eval_fn[1]}};
-    HIP_OK(hipEventRecord(c->ev[0], c->stream));
-    if (launch_frame(a, fl, c->stream, c->ev[2], c->ev[3]) != 0) return TSI_E_HIP;
-    HIP_OK(hipEventRecord(c->ev[1], c->stream));
-    HIP_OK(hipEventSynchronize(c->ev[1]));
-    Counters h;
-    HIP_OK(hipMemcpy(&h, c->cnt, sizeof h, hipMemcpyDeviceToHost));
+    HIP_OK(hipEventRecord(f.begin, c->stream));
+    if (launch_frame(a, fl, c->stream, f.eval0, f.eval1) != 0) return TSI_E_HIP;
+    HIP_OK(hipMemcpyAsync(f.h, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipEventRecord(f.end, c->stream));
+    c->f_issued += 1;
+    return TSI_OK;
+}
+
+int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
+    if (!c) return TSI_E_ARG;
+    if (c->f_done == c->f_issued) return TSI_E_ARG;   // no frame in flight
+    tsi_ctx::FrameSlot &f = c->fslot[c->f_done % tsi_ctx::kFrameSlots];
+    c->f_done += 1;
+    HIP_OK(hipEventSynchronize(f.end));
+    const Counters &h = *f.h;
     float ms = 0.0f, ems = 0.0f;
-    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-    hipEventElapsedTime(&ems, c->ev[2], c->ev[3]);
+    hipEventElapsedTime(&ms, f.begin, f.end);
+    hipEventElapsedTime(&ems, f.eval0, f.eval1);
     if (gpu_ms) *gpu_ms = ms;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
@@ -488,22 +534,36 @@ int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
     return TSI_OK;
 }
 
+int tsi_set_inversion(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
+    if (!c) return TSI_E_ARG;
+    if (c->f_issued != c->f_done) return TSI_E_PENDING;
+    int rc = tsi_set_inversion_async(c);
+    if (rc) return rc;
+    return tsi_set_inversion_result(c, gpu_ms, stats);
+}
+
 namespace {
 
-// Waits for the pending asynchronous trace and records its device time and node visits.
+// Waits for the oldest trace in flight and records its device time and node visits.
 int harvest_trace(tsi_ctx *c) {
-    if (!c->trace_pending) return TSI_OK;
-    c->trace_pending = false;
-    HIP_OK(hipEventSynchronize(c->ev_tdone));
-    unsigned long long v = 0;
-    HIP_OK(hipMemcpy(&v, c->visits, sizeof v, hipMemcpyDeviceToHost));
+    if (c->t_done == c->t_issued) return TSI_OK;
+    tsi_ctx::TraceSlot &t = c->tslot[c->t_done % tsi_ctx::kTraceSlots];
+    c->t_done += 1;
+    HIP_OK(hipEventSynchronize(t.end));
     float tms = 0.0f;
-    hipEventElapsedTime(&tms, c->ev[4], c->ev_tdone);
-    c->last_visits = v;
+    hipEventElapsedTime(&tms, t.begin, t.end);
+    c->last_visits = *t.visits_h;
     c->last_trace_ms = tms;
     c->trace_ms_total += tms;
-    c->trace_visits_total += v;
+    c->trace_visits_total += *t.visits_h;
     c->trace_count += 1;
+    return TSI_OK;
+}
+
+int harvest_all(tsi_ctx *c) {
+    int rc;
+    while (c->t_done != c->t_issued)
+        if ((rc = harvest_trace(c))) return rc;
     return TSI_OK;
 }
 
@@ -514,29 +574,30 @@ int tsi_trace_async(tsi_ctx *c) {
     const uint32_t W = c->u.resolution[0], H = c->u.resolution[1];
     if (W == 0 || H == 0) return TSI_E_ARG;
     int rc;
-    if ((rc = harvest_trace(c))) return rc;   // the snapshot and frame buffers are free again
+    if (c->t_issued - c->t_done >= tsi_ctx::kTraceSlots && (rc = harvest_trace(c))) return rc;
     const size_t px = size_t(W) * H;
-    if (px > c->frame_px) {
+    if (px > c->frame_px) {   // the traces in flight write the old buffers
+        if ((rc = harvest_all(c))) return rc;
         dfree(c->rgba);
         dfree(c->depth);
         c->frame_px = 0;
         if ((rc = dmalloc(&c->rgba, px)) || (rc = dmalloc(&c->depth, px))) return rc;
         c->frame_px = px;
     }
-    const uint64_t used = c->used_nodes;
-    if (used > c->snap_cap) {
-        uint64_t cap = used * 2u < c->nodes ? used * 2u : c->nodes;
-        if (cap < used) cap = used;
-        dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
-        c->snap_cap = 0;
-        if ((rc = dmalloc(&c->snap_st, cap)) || (rc = dmalloc(&c->snap_blo, cap)) || (rc = dmalloc(&c->snap_bhi, cap)))
+    if (!c->snap_st) {
+        if ((rc = dmalloc(&c->snap_st, c->nodes)) || (rc = dmalloc(&c->snap_blo, c->nodes)) ||
+            (rc = dmalloc(&c->snap_bhi, c->nodes))) {
+            dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
             return rc;
-        c->snap_cap = cap;
+        }
     }
+    // the snapshot is rewritten only after the previous trace has read it (a device-side wait)
+    if (c->t_issued != 0)
+        HIP_OK(hipStreamWaitEvent(c->stream, c->tslot[(c->t_issued - 1) % tsi_ctx::kTraceSlots].end, 0));
+    KArgs a = make_args(c);
     // snapshot of the nodes in use, in stream order after the last set inversion
-    HIP_OK(hipMemcpyAsync(c->snap_st, c->st, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->snap_blo, c->blo, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->snap_bhi, c->bhi, used * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    if (launch_snapshot(a, c->snap_st, c->snap_blo, c->snap_bhi, c->nodes, c->grid_misc, c->stream) != 0)
+        return TSI_E_HIP;
     HIP_OK(hipEventRecord(c->ev_snap, c->stream));
     HIP_OK(hipStreamWaitEvent(c->tstream, c->ev_snap, 0));
     TraceArgs t{};
@@ -548,22 +609,23 @@ int tsi_trace_async(tsi_ctx *c) {
     t.rgba = c->rgba;
     t.depth = c->depth;
     t.visits = c->visits;
-    KArgs a = make_args(c);
     a.st = c->snap_st;
     a.blo = c->snap_blo;
     a.bhi = c->snap_bhi;
+    tsi_ctx::TraceSlot &ts = c->tslot[c->t_issued % tsi_ctx::kTraceSlots];
     HIP_OK(hipMemsetAsync(c->visits, 0, sizeof(unsigned long long), c->tstream));
-    HIP_OK(hipEventRecord(c->ev[4], c->tstream));
+    HIP_OK(hipEventRecord(ts.begin, c->tstream));
     if (launch_trace(a, t, c->tstream) != 0) return TSI_E_HIP;
-    HIP_OK(hipEventRecord(c->ev_tdone, c->tstream));
-    c->trace_pending = true;
+    HIP_OK(hipMemcpyAsync(ts.visits_h, c->visits, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->tstream));
+    HIP_OK(hipEventRecord(ts.end, c->tstream));
+    c->t_issued += 1;
     return TSI_OK;
 }
 
 int tsi_trace_wait(tsi_ctx *c, float *rgba_out, float *depth_out) {
     if (!c) return TSI_E_ARG;
     int rc;
-    if ((rc = harvest_trace(c))) return rc;
+    if ((rc = harvest_all(c))) return rc;
     const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
     if ((rgba_out || depth_out) && px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
     if (rgba_out) HIP_OK(hipMemcpy(rgba_out, c->rgba, px * sizeof(float4), hipMemcpyDeviceToHost));
@@ -588,7 +650,7 @@ int tsi_trace_totals(tsi_ctx *c, double *trace_ms, uint64_t *node_visits, uint64
 int tsi_frame_rgba8(tsi_ctx *c, uint8_t *rgba8_out) {
     if (!c || !rgba8_out) return TSI_E_ARG;
     int rc;
-    if ((rc = harvest_trace(c))) return rc;
+    if ((rc = harvest_all(c))) return rc;
     const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
     if (px == 0 || px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
     uchar4 *tmp = nullptr;

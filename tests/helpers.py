@@ -36,7 +36,7 @@ def run_pair(tape, times, width=0, height=0, trace_every=0, voxel_capacity=1 << 
     """Runs the same frame sequence on the GPU library and on the oracle; yields per frame
     (k, gpu_nodes, oracle_nodes, gpu_stats, oracle_stats, gpu_frame, oracle_frame)."""
     fr = oracle.Frame(tape, voxel_capacity=voxel_capacity,
-                      **{k: v for k, v in params.items() if k not in ("prune", "engine")})
+                      **{k: v for k, v in params.items() if k not in ("prune", "engine", "prune_live_max")})
     st = T.State(tape, width=max(width, 16), height=max(height, 16), voxel_capacity=voxel_capacity, **params)
     try:
         st.reset()
