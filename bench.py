@@ -116,9 +116,12 @@ def main():
                     help="tape evaluation: compiled straight-line kernels (default) or the interpreter")
     ap.add_argument("--sync-trace", action="store_true",
                     help="trace each frame synchronously instead of overlapping it with the next set inversion")
-    ap.add_argument("--prune", default="on", choices=["on", "off", "exact"],
-                    help="pruned per-group programs: on (default), off (full tape per cell), exact "
-                         "(only for tapes whose pruned results are provably identical)")
+    ap.add_argument("--prune", default="auto", choices=["auto", "on", "off", "exact"],
+                    help="liveness-pruned evaluation: on, off (full tape per cell), exact (only for "
+                         "tapes whose pruned results are provably identical), auto (default: on for "
+                         "swept-volume workloads, where every cell's t-window lies inside its "
+                         "parent's, off otherwise, where most reverse-list cells cannot use their "
+                         "parent's record and the bookkeeping costs more than it saves)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="instead of the benchmark: copy a buffer of this many MiB device-to-device 5 times "
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -158,10 +161,11 @@ def main():
     import temporalsetinversion_amd as T
 
     tape = T.tape_from_ll(ll)
+    prune = args.prune if args.prune != "auto" else ("on" if w["swept_volume"] else "off")
     st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
                  point_sample=w["point_sample"], swept_volume=w["swept_volume"],
                  lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27,
-                 prune={"on": True, "off": False, "exact": "exact"}[args.prune], engine=args.engine)
+                 prune={"on": True, "off": False, "exact": "exact"}[prune], engine=args.engine)
     info = st.tape_info()
     st.reset()
     frame = 0
@@ -250,7 +254,7 @@ def main():
                    "lipschitz_corrected": w["lipschitz_corrected"], "swept_volume": w["swept_volume"],
                    "point_sample": w["point_sample"], "times": "t_k = k/60", "engine": info["engine"],
                    "trace": "synchronous" if args.sync_trace else "overlapped with the next set inversion",
-                   "prune": args.prune, "frames_in_flight": 1 if args.sync_trace else 2},
+                   "prune": prune, "frames_in_flight": 1 if args.sync_trace else 2},
         "tape_compile_s": info["compile_ms"] / 1e3, "tape_compile_cached": bool(info["cache_hit"]),
         "records_run_per_cell": (len(tape) - 4) - tot["pruned"] / max(tot["cells"], 1),
         "set_inversion_ms_per_frame": tot["gpu_ms"] / args.steps,

@@ -271,6 +271,13 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
     uint32_t region = 0u;
     V3 p0, p1;
     box(region, p0, p1);
+    // q0/q1: the stored (unwidened) box of the node whose box p0/p1 was last set from: a child's
+    // box is computed from it exactly as the set inversion computed the stored one
+    // (lo + d * octant, d = (hi - lo) / 2), so the children's boxes are not loaded
+    V3 q0 = p0, q1 = p1;
+    uint32_t qnode = 0u;   // the node q0/q1 belong to
+    // meta of the node entered by the last descent (its st was just read)
+    uint32_t entered = FAIL, entered_meta = 0u;
     V3 hit = efficient_slabs(p0, p1, o2, inv);
     if (hit.x == 0.0f) { col = c; dist = -1.0f; return; }
     float sms = 0.0f;
@@ -283,6 +290,9 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
             const uint32_t top = peak();
             if (top != FAIL) {
                 box(top, p0, p1);
+                q0 = p0;
+                q1 = p1;
+                qnode = top;
                 hit = efficient_slabs(p0, p1, o2, inv);
             }
             continue;
@@ -301,7 +311,12 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
         float sdz = ((sgz * (float(mz) - roz)) + sgz * 0.5f + 0.5f) * ddz;
         float mkx = 0.0f, mky = 0.0f, mkz = 0.0f;
         const V3 old = o2;
-        const uint32_t rmeta = meta_of(a.st[region]) & kChildMask;
+        const uint32_t rmeta = (region == entered ? entered_meta : meta_of(a.st[region])) & kChildMask;
+        entered = FAIL;
+        // (after a DDA that left the 2x2x2 grid the popped node keeps the previous box, as in the
+        // reference; its children's boxes are then loaded)
+        const bool qown = qnode == region;
+        const float hx = (q1.x - q0.x) / 2.0f, hy = (q1.y - q0.y) / 2.0f, hz = (q1.z - q0.z) / 2.0f;
         for (int i = 0; i < 32; ++i) {
             if (wmax(p1.x - p0.x, wmax(p1.y - p0.y, p1.z - p0.z)) < 1.0f) {
                 c.x = c.x * kDarken; c.y = c.y * kDarken; c.z = c.z * kDarken;
@@ -314,6 +329,9 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
                 const uint32_t top = peak();
                 if (top != FAIL) {
                     box(top, p0, p1);
+                    q0 = p0;
+                    q1 = p1;
+                    qnode = top;
                     hit = efficient_slabs(p0, p1, o2, inv);
                 }
                 break;
@@ -325,7 +343,12 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
                 const float4 cs = a.st[child];
                 ++visits;
                 if (cs.x <= 0.0f) {
-                    const float4 lo = a.blo[child], hi = a.bhi[child];
+                    // the child's box as eval_level stored it (tsi_eval.h), from its parent's
+                    const float ox = float(uint32_t(ci) & 1u), oy = float((uint32_t(ci) >> 1) & 1u),
+                                oz = float(uint32_t(ci) >> 2);
+                    V3 lo = v3(q0.x + hx * ox, q0.y + hy * oy, q0.z + hz * oz);
+                    V3 hi = v3(q0.x + hx * (1.0f + ox), q0.y + hy * (1.0f + oy), q0.z + hz * (1.0f + oz));
+                    if (!qown) box(child, lo, hi);
                     const V3 tp0 = v3(lo.x - kTrEps, lo.y - kTrEps, lo.z - kTrEps);
                     const V3 tp1 = v3(hi.x + kTrEps, hi.y + kTrEps, hi.z + kTrEps);
                     const V3 th = efficient_slabs(tp0, tp1, o2, inv);
@@ -335,6 +358,11 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
                             push(child);
                             p0 = tp0;
                             p1 = tp1;
+                            q0 = lo;
+                            q1 = hi;
+                            qnode = child;
+                            entered = child;
+                            entered_meta = meta_of(cs);
                             hit = th;
                             break;
                         } else {
