@@ -63,6 +63,67 @@ def profiled_traffic(workload: str, engine: str):
     return sum(k * b for k, b in rows.values()) / n, os.path.relpath(path, ROOT)
 
 
+def _short(name: str) -> str:
+    return name.split("(")[0]
+
+
+def summarize_profile(args) -> None:
+    """Prints the committed profile summary (profiles/tape_eval_<workload>.txt) from the
+    rocprofv3 databases of the kernel-trace run, the FETCH_SIZE / WRITE_SIZE runs of the same
+    workload and the same two counters over a 256 MiB device copy (the scale that turns the
+    counters into bytes on this machine)."""
+    import sqlite3
+
+    def per_kernel(db, sql):
+        out = {}
+        for name, n, v in sqlite3.connect(db).execute(sql):
+            k = _short(name)
+            m, t = out.get(k, (0, 0.0))
+            out[k] = (m + n, t + v)
+        return out
+
+    kt = per_kernel(args.summarize, "select name, count(*), sum(duration) from kernels group by name")
+    total = sum(t for _, t in kt.values())
+    frames = kt.get("tsi::k_frame_begin", (0, 0))[0]
+    W, R = args.workload, "$R"
+    tag = {"fetch": args.pmc_fetch, "write": args.pmc_write}
+    lines = [f"# Kernel time and HBM traffic: {W}, compiled tape engine, bench defaults",
+             "#",
+             "# commands (run from /tmp on the GPU host, repo at $R; <d> = an output directory):",
+             f"#   rocprofv3 --kernel-trace --stats -d <d>/kt -o kt -- python3 {R}/bench.py --steps 5 --warmup 2 --no-cpu --workload {W}"]
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        lines.append(f"#   rocprofv3 --pmc {c} -d <d>/{c} -o pmc -- python3 {R}/bench.py --steps 2 --warmup 1 --no-cpu --workload {W}")
+        lines.append(f"#   rocprofv3 --pmc {c} -d <d>/copy_{c} -o pmc -- python3 {R}/bench.py --calibrate-copy-mib {args.copy_mib}")
+    lines.append(f"#   python3 bench.py --workload {W} --summarize <d>/kt/kt_results.db --pmc-fetch <d>/FETCH_SIZE/pmc_results.db "
+                 "--pmc-write <d>/WRITE_SIZE/pmc_results.db --copy-fetch <d>/copy_FETCH_SIZE/pmc_results.db "
+                 "--copy-write <d>/copy_WRITE_SIZE/pmc_results.db")
+    lines += ["#", f"# {frames} frames (warm-up + timed); one forward-only and one reverse level kernel per octree",
+              "# level (tsi_tape_eval_fwd / _rev), one snapshot + trace per frame on the second stream.", ""]
+    lines.append(f"{'kernel':<44} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
+    for k, (n, t) in sorted(kt.items(), key=lambda x: -x[1][1]):
+        lines.append(f"{k[:44]:<44} {n:>10} {t / 1e6:>10.3f} {t / n / 1e3:>9.2f} {100 * t / total:>6.1f}%")
+    if args.pmc_fetch and args.pmc_write and args.copy_fetch and args.copy_write:
+        q = "select name, count(*), sum(counter_value) from pmc_events group by name"
+        kib = args.copy_mib * 1024.0
+
+        def scale(db):   # KiB the counter reports per copy dispatch -> bytes
+            n, v = per_kernel(db, q)["__amd_rocclr_copyBuffer"]
+            return kib / (v / n)
+
+        sf, sw = scale(args.copy_fetch), scale(args.copy_write)
+        f, w = per_kernel(args.pmc_fetch, q), per_kernel(args.pmc_write, q)
+        lines += ["", f"copy calibration ({args.copy_mib} MiB device-to-device): FETCH_SIZE scale {sf:.3f}, "
+                      f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024)",
+                  f"{'kernel':<24} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'hbm_bytes/disp':>15}"]
+        for k in ("tsi_tape_eval_fwd", "tsi_tape_eval_rev", "tsi::k_trace", "tsi::k_snapshot"):
+            if k in f and k in w:
+                nf, vf = f[k]
+                nw, vw = w[k]
+                b = (vf / nf * sf + vw / nw * sw) * 1024.0
+                lines.append(f"{k:<24} {nf:>10} {vf / nf:>15.1f} {vw / nw:>15.1f} {b:>15.0f}")
+    print("\n".join(lines))
+
+
 def _cpu_worker(job):
     """One oracle frame sequence (frames t_k = (k + phase)/60) until the time budget is spent."""
     import oracle

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,21 @@ bool is_minmax_form(uint32_t form) {
     return form == F_MIN_RR || form == F_MAX_RR || form == F_MIN_RL || form == F_MAX_RL;
 }
 
+// forms whose reverse rule needs only a decision (compute.wgsl 2640 ABS_I_RR, 2884-2903 MIN/MAX)
+bool is_code_form(uint32_t form) { return is_minmax_form(form) || form == F_ABS; }
+
+// operand values the reverse rule of a form reads (rev_form_t in tsi_rules.h): x = the first
+// register operand (the register of a literal-then-register form), y = the second
+void rev_needs(uint32_t form, bool &x, bool &y) {
+    switch (form) {
+    case F_MUL_RR: case F_DIV_RR: case F_POW_RR: case F_ATAN2_RR: x = y = true; return;
+    case F_SQR: case F_SQRT: case F_SIN: case F_COS: case F_EXP: case F_LOG: case F_FLOOR:
+    case F_POW_RL: case F_ATAN2_RL: case F_DIV_LR:
+        x = true; y = false; return;
+    default: x = y = false; return;
+    }
+}
+
 JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L) {
     JitLayout y;
     y.group = region_records();
@@ -136,6 +152,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
          "TSI_DI Iv fixk(Iv x, FixK k) {   // constant operands keep fix() so that it folds\n"
          "    if (__builtin_constant_p(x.lo) && __builtin_constant_p(x.hi)) return fix(x);\n"
          "    return I(wmax(x.lo, k.lo), wmin(x.hi, k.hi));\n}\n";
+    o += "TSI_DI uint32_t abs_code(Iv x) { return I_CONTAINS_ZERO(x) ? 3u : (x.hi < 0.0f ? 2u : 1u); }\n";
     o += "struct FwdOut { Iv field; uint32_t cost, records; };\n";
     o += "struct RevOut { Iv s0, s1, s2, s3; uint32_t ops; };\n\n";
 
@@ -157,6 +174,43 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // The tape functions are calls, not inlined into the level kernel: inlined, the register
     // allocation of the kernel's own state and of the tape interfere (several times the
     // registers of either alone, with spills); a call costs a few instructions per wavefront.
+    // ---- the saved-operand stack of the reverse kernel ----
+    // The reverse rules read operand values (MUL_RR, DIV, SQR, SQRT, the transcendental forms) or
+    // only a decision (MIN/MAX: which operands receive [1, 1]; ABS: the sign class).  Each
+    // value needed is saved once, when it is computed (row slot[k]); decisions are 2-bit codes,
+    // 16 to a word, a word saved when its last code is set (row nslots + w).
+    std::vector<int> slot(n, -1), cidx(n, -1);
+    uint32_t nslots = 0, ncodes = 0;
+    {
+        std::vector<char> need(n, 0);
+        for (uint32_t j = 4; j < n; ++j) {
+            const uint32_t side = side_of(tape[j]);
+            if (side == 3) continue;
+            const uint32_t form = fwd[j - 4].form;
+            if (is_code_form(form)) { cidx[j] = int(ncodes++); continue; }
+            bool nx = false, ny = false;
+            rev_needs(form, nx, ny);
+            if (nx) need[side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])] = 1;
+            if (ny) need[rarg_of(tape[j])] = 1;
+        }
+        for (uint32_t k = 0; k < n; ++k)
+            if (need[k]) slot[k] = int(nslots++);
+    }
+    const uint32_t code_words = (ncodes + 15u) / 16u;
+    // the region after which code word w is complete
+    std::vector<std::vector<uint32_t>> flush_after(Y.regions + 1);
+    for (uint32_t j = 4; j < n; ++j)
+        if (cidx[j] >= 0 && (cidx[j] % 16 == 15 || uint32_t(cidx[j]) == ncodes - 1u))
+            flush_after[region_of(j)].push_back(uint32_t(cidx[j]) / 16u);
+    if (rev_kernel && nslots + code_words > L.stack + 1u)
+        throw std::runtime_error("saved-operand stack layout exceeds the allocation");
+    auto code_expr = [&](uint32_t j, const std::string &l, const std::string &rhs) {
+        const uint32_t form = fwd[j - 4].form;
+        if (form == F_ABS) return "abs_code(" + l + ")";
+        const bool is_min = form == F_MIN_RR || form == F_MIN_RL;
+        return std::string(is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ")";
+    };
+
     // choices: the MIN/MAX choices go to chbuf (always with pr)
     auto emit_forward = [&](const std::string &fname, bool choices) {
     o += "template <bool PUSH>\n__device__ __noinline__ FwdOut " + fname + "(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
@@ -166,6 +220,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     if (choices)   // skipped regions leave their MIN/MAX undecided (3)
         for (uint32_t w = 0; w < MW; ++w) o += "    uint32_t cw" + std::to_string(w) + (pr ? " = 0xFFFFFFFFu;\n" : " = 0u;\n");
     for (uint32_t j = 4; j < n; ++j) o += "    Iv " + V(j) + (pr ? " = ZI;\n" : ";\n");
+    for (uint32_t w = 0; w < code_words; ++w) o += "    uint32_t kw" + std::to_string(w) + " = 0u;\n";
+    for (uint32_t k = 0; k < 4; ++k)
+        if (slot[k] >= 0) o += "    if (PUSH) " + P(uint32_t(slot[k])) + " = make_float2(" + V(k) + ".lo, " + V(k) + ".hi);\n";
     for (uint32_t j = 4; j < n; ++j) {
         const auto &g = tape[j];
         const FwdIns &f = fwd[j - 4];
@@ -192,9 +249,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                  ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "))), K);\n";
             const int m = mmx[j];
             const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
-            o += "        { const uint32_t ch = md" + J + " != 3u ? md" + J + " : " +
+            o += "        const uint32_t ch" + J + " = md" + J + " != 3u ? md" + J + " : " +
                  (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ");\n";
-            o += "          cw" + w + " = (cw" + w + " & ~(3u << " + sh + ")) | (ch << " + sh + "); }\n";
+            o += "        cw" + w + " = (cw" + w + " & ~(3u << " + sh + ")) | (ch" + J + " << " + sh + ");\n";
         } else if (choices && mmx[j] >= 0) {
             const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
             const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
@@ -206,18 +263,23 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         } else {
             o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
         }
-        const uint32_t npush = f.npush_side & 3u;
-        if (npush) {
-            o += "        if (PUSH) { ";
-            if (npush == 3) {
-                o += P(f.push) + " = make_float2(" + rr + ".lo, " + rr + ".hi); ";
-            } else {
-                o += P(f.push) + " = make_float2(" + l + ".lo, " + l + ".hi); ";
-                if (npush == 2) o += P(f.push + 1u) + " = make_float2(" + rr + ".lo, " + rr + ".hi); ";
-            }
-            o += "}\n";
+        if (slot[j] >= 0)
+            o += "        if (PUSH) " + P(uint32_t(slot[j])) + " = make_float2(" + V(j) + ".lo, " + V(j) + ".hi);\n";
+        if (cidx[j] >= 0) {
+            const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
+            const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
+            // a decided MIN/MAX of the pruned variant keeps the parent's choice (its code is
+            // not read: the reverse pass takes the mode)
+            const std::string code = (pr && mmx[j] >= 0) ? "ch" + std::to_string(j) : code_expr(j, l, rhs);
+            o += "        if (PUSH) kw" + std::to_string(cidx[j] / 16) + " |= " + code + " << " +
+                 std::to_string(2 * (cidx[j] % 16)) + "u;\n";
         }
-        if ((j - 4) % G == G - 1 || j == n - 1) o += close_region();
+        if ((j - 4) % G == G - 1 || j == n - 1) {
+            o += close_region();
+            for (uint32_t w : flush_after[region_of(j)])
+                o += "    if (PUSH) " + P(nslots + w) + " = make_float2(__uint_as_float(kw" + std::to_string(w) +
+                     "), 0.0f);\n";
+        }
     }
     if (choices)
         for (uint32_t w = 0; w < MW; ++w)
@@ -240,21 +302,45 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const uint32_t side = side_of(g);
             if (side == 0 || side == 1) touched[larg_of(g)] = 1;
             if (side == 0 || side == 2) touched[rarg_of(g)] = 1;
-            const uint32_t npush = fwd[j - 4].npush_side & 3u;
-            if (side != 3 && npush) o += "    float2 p_" + std::to_string(j) + ";\n";
-            if (side != 3 && npush == 2) o += "    float2 q_" + std::to_string(j) + ";\n";
         }
+        for (uint32_t k = 0; k < n; ++k)
+            if (slot[k] >= 0) o += "    float2 sv" + std::to_string(k) + ";\n";
+        for (uint32_t w = 0; w < code_words; ++w) o += "    uint32_t kr" + std::to_string(w) + ";\n";
         for (uint32_t k = 0; k < n; ++k)
             if (touched[k]) o += "    Iv " + S(k) + (pr ? " = ZI;\n" : ";\n");
         uint64_t static_ops = 0;
+        // operand x / y of record j in the reverse pass: its saved value, or [+0, +0] when the
+        // pruned forward pass skipped the value's region (as the value itself was then)
+        auto operand_of = [&](uint32_t j, bool second) -> int {
+            const uint32_t side = side_of(tape[j]);
+            bool nx = false, ny = false;
+            rev_needs(fwd[j - 4].form, nx, ny);
+            if (!second) return nx ? int(side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])) : -1;
+            return ny ? int(rarg_of(tape[j])) : -1;
+        };
+        auto saved = [&](uint32_t j, int k) -> std::string {
+            if (k < 0) return "ZI";
+            const std::string v = "I(sv" + std::to_string(k) + ".x, sv" + std::to_string(k) + ".y)";
+            if (pr && k >= 4 && region_of(uint32_t(k)) != region_of(j))
+                return "(" + bit(region_of(uint32_t(k))) + " ? " + v + " : ZI)";
+            return v;
+        };
         auto emit_loads = [&](uint32_t hi, uint32_t lo) {   // saved operands of records hi-1 .. lo
+            std::vector<char> vdone(n, 0), wdone(code_words + 1, 0);
             for (uint32_t jj = hi; jj-- > lo;) {
                 if (side_of(tape[jj]) == 3) continue;
-                const FwdIns &f = fwd[jj - 4];
-                const uint32_t npush = f.npush_side & 3u;
-                const std::string J = std::to_string(jj);
-                if (npush) o += "        p_" + J + " = " + P(f.push) + ";\n";
-                if (npush == 2) o += "        q_" + J + " = " + P(f.push + 1u) + ";\n";
+                if (cidx[jj] >= 0) {
+                    const uint32_t w = uint32_t(cidx[jj]) / 16u;
+                    if (!wdone[w])
+                        o += "        kr" + std::to_string(w) + " = __float_as_uint(" + P(nslots + w) + ".x);\n";
+                    wdone[w] = 1;
+                    continue;
+                }
+                for (bool second : {false, true}) {
+                    const int k = operand_of(jj, second);
+                    if (k >= 0 && !vdone[k]) o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
+                    if (k >= 0) vdone[k] = 1;
+                }
             }
         };
         // regions from the top: records [hi, lo) with lo the region start
@@ -290,21 +376,31 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     if (j == n - 1) { sj = "ONE"; sj0 = "false"; }
                     else if (live[j]) { sj = S(j); sj0 = "(" + S(j) + ".lo == 0.0f && " + S(j) + ".hi == 0.0f)"; }
                     else { sj = "ZI"; sj0 = "true"; }
-                    const uint32_t npush = f.npush_side & 3u;
-                    const std::string x = npush ? "I(p" + J + ".x, p" + J + ".y)" : "ZI";
-                    const std::string y = npush == 2 ? "I(q" + J + ".x, q" + J + ".y)" : "ZI";
+                    const std::string x = saved(j, operand_of(j, false)), y = saved(j, operand_of(j, true));
                     const std::string form = std::to_string(f.form) + "u";
                     const bool mm = pr && mmx[j] >= 0;
+                    // the rule's derivatives: from the saved values, or from the decision code
+                    std::string rule = "rev_form_t<true>(" + form + ", " + x + ", " + y + ", " + lit(g.literal) + ")";
+                    if (cidx[j] >= 0) {
+                        const uint32_t w = uint32_t(cidx[j]) / 16u;
+                        o += "        const uint32_t kc" + J + " = (kr" + std::to_string(w) + " >> " +
+                             std::to_string(2 * (cidx[j] % 16)) + "u) & 3u;\n";
+                        const std::string kc = "kc" + J;
+                        if (f.form == F_ABS)
+                            rule = "Iv2{sel(" + kc + " == 3u, I(-1.0f, 1.0f), sel(" + kc + " == 2u, I_FROM(-1.0f), ONE)), ZI}";
+                        else if (f.form == F_MIN_RL || f.form == F_MAX_RL)
+                            rule = "Iv2{sel((" + kc + " & 1u) != 0u, ONE, ZI), ZI}";
+                        else
+                            rule = "Iv2{sel((" + kc + " & 1u) != 0u, ONE, ZI), sel((" + kc + " & 2u) != 0u, ONE, ZI)}";
+                    }
                     if (mm) {
                         o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
-                        o += "        Iv2 d" + J + " = rev_form_t<true>(" + form + ", " + x + ", " + y + ", " +
-                             lit(g.literal) + ");\n";
+                        o += "        Iv2 d" + J + " = " + rule + ";\n";
                         // decided: the kept operand receives [1, 1] (rev_step)
                         o += "        d" + J + ".a = sel(md" + J + " != 3u, ONE, d" + J + ".a);\n";
                         o += "        d" + J + ".b = sel(md" + J + " != 3u, ONE, d" + J + ".b);\n";
                     } else {
-                        o += "        const Iv2 d" + J + " = rev_form_t<true>(" + form + ", " + x + ", " + y + ", " +
-                             lit(g.literal) + ");\n";
+                        o += "        const Iv2 d" + J + " = " + rule + ";\n";
                     }
                     const std::string cost = std::to_string(unsigned(rv.cost)) + "u";
                     // MUL_RR's derivatives are the saved operand values; a record's value
@@ -313,6 +409,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     auto fixed_d = [&](const char *dx) {
                         const std::string d = "d" + J + "." + dx;
                         if (f.form == F_MUL_RR && (dx[0] == 'a' ? b : a) >= 4u) return d;
+                        if (cidx[j] >= 0) return d;   // 0, 1 or -1: fix leaves them
                         return "fixk(" + d + ", K)";
                     };
                     // the first contribution to a sensitivity is not added to [+0, +0]: that

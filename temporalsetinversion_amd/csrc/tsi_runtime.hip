@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -238,6 +239,8 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
         std::thread t([&] { rc[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
         rc[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
         t.join();
+    } catch (const std::runtime_error &) {   // the generator cannot lay out this tape
+        return TSI_E_JIT;
     } catch (...) {
         return TSI_E_OOM;
     }
@@ -729,6 +732,8 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t fl
                 }
             }
         }
+    } catch (const std::runtime_error &) {
+        return TSI_E_JIT;
     } catch (...) {
         return TSI_E_OOM;
     }
