@@ -49,18 +49,22 @@ def profiled_traffic(workload: str, engine: str):
     (FETCH_SIZE/WRITE_SIZE passes scaled by the copy calibration; see profiles/)."""
     path = os.path.join(ROOT, "profiles", f"tape_eval_{workload.split('_')[0]}_{workload.split('_', 1)[1]}.txt")
     if engine != "compiled" or not os.path.exists(path):
-        return None, None
-    rows = {}
+        return None, None, None
+    rows, times = {}, {}
     in_traffic = False
     for line in open(path):
         parts = line.split()
         in_traffic = in_traffic or "hbm_bytes/disp" in line
         if in_traffic and len(parts) == 5 and parts[0].startswith("tsi_tape_eval_"):
             rows[parts[0]] = (int(parts[1]), float(parts[4]))
+        elif not in_traffic and len(parts) == 5 and parts[0].startswith("tsi_tape_eval_"):
+            times[parts[0]] = (int(parts[1]), float(parts[3]))   # dispatches, avg_us
     if not rows:
-        return None, None
+        return None, None, None
     n = sum(k for k, _ in rows.values())
-    return sum(k * b for k, b in rows.values()) / n, os.path.relpath(path, ROOT)
+    m = sum(k for k, _ in times.values())
+    avg_us = sum(k * t for k, t in times.values()) / m if m else None
+    return sum(k * b for k, b in rows.values()) / n, os.path.relpath(path, ROOT), avg_us
 
 
 def _short(name: str) -> str:
@@ -292,7 +296,7 @@ def main():
     ops = tot["fwd_ops"] + tot["rev_ops"]
     achieved = ops / (tot["eval_ms"] * 1e-3) / 1e12 if tot["eval_ms"] > 0 else 0.0
     launches = max(tot["launches"], 1)
-    traffic, traffic_src = profiled_traffic(args.workload, info["engine"])
+    traffic, traffic_src, prof_avg_us = profiled_traffic(args.workload, info["engine"])
     # trace: SURVEY-style algorithmic bytes, 40 B per node visit + 20 B per pixel (colour + depth)
     px = w["width"] * w["height"]
     trace_bytes = trace["visits"] * 40 + args.steps * px * 20
@@ -329,7 +333,10 @@ def main():
                      "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic, "traffic_unit": "HBM bytes per launch",
                      "traffic_source": traffic_src,
                      "algorithmic_ops_per_launch": ops / launches,
+                     # HIP events around the level loop: kernels plus the dispatch gaps between them
                      "avg_launch_us": tot["eval_ms"] * 1e3 / launches,
+                     # the same kernels' average duration in the committed rocprofv3 summary
+                     "profile_avg_kernel_us": prof_avg_us,
                      "peak_unpacked": PEAK_VALU_UNPACKED_TOPS,
                      "eval_ms_per_frame": tot["eval_ms"] / args.steps,
                      "ops_per_frame": ops / args.steps},
