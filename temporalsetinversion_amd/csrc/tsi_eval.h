@@ -29,13 +29,36 @@ typedef float Slots[kSlots];
 namespace tsi {
 
 
+// One step of the dealloc cascade (compute.wgsl:3533-3543) over list `step` (buffer step & 1):
+// every node of a freed block that has children passes its child block on to list step + 1,
+// and the block goes to the pending free list.  A wavefront takes 8 blocks (lane = 8 * block
+// slot + node).  Returns this lane's count of freed blocks.
+TSI_DI uint64_t dealloc_step(const KArgs &a, int step, uint32_t wave, uint32_t nwaves) {
+    Counters *cnt = a.cnt;
+    const uint32_t count = min(cnt->dlist_count[step], a.dlist_cap);
+    const uint32_t *in = a.dlists[step & 1];
+    uint32_t *out = a.dlists[(step + 1) & 1];
+    const uint32_t lane = lane_id();
+    const uint32_t o = lane & 7u;
+    uint64_t freed = 0;
+    for (uint32_t g = wave; g * 8u < count; g += nwaves) {
+        const uint32_t idx = g * 8u + (lane >> 3);
+        const bool active = idx < count;
+        const uint32_t b = active ? in[idx] : 0u;
+        const uint32_t m = active ? meta_of(a.st[b * 8u + o]) & kChildMask : 0u;
+        wave_append(m != 0, m - 1u, out, &cnt->dlist_count[step + 1], a.dlist_cap, &cnt->frame_error, kErrQueue);
+        wave_append(active && o == 0, b, a.pending_free, &cnt->pending, a.nblocks, &cnt->frame_error, kErrQueue);
+        freed += (active && o == 0) ? 1 : 0;
+    }
+    return freed;
+}
+
 template <bool REV, class Engine>
 TSI_DI void eval_level(const KArgs &a, int level, float time) {
     const int which = REV ? 1 : 0;
     Counters *cnt = a.cnt;
     // a list that overflowed (flagged by wave_append) holds list_cap valid entries
     const uint32_t count = min(cnt->list_count[level][which], a.list_cap);
-    if (count == 0) return;
     const uint32_t *list = a.lists[level & 1][which];
     uint32_t *next_f = a.lists[(level + 1) & 1][0];
     uint32_t *next_r = a.lists[(level + 1) & 1][1];
@@ -242,7 +265,9 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
                                           a.list_cap, &cnt->frame_error, kErrQueue);
         const uint32_t at_r = wave_append(enq && !next_fwd_only, C, next_r, &cnt->list_count[level + 1][1],
                                           a.list_cap, &cnt->frame_error, kErrQueue);
-        wave_append(drop_children, old_child - 1u, a.dlists[0], &cnt->dlist_count[0], a.dlist_cap,
+        // dropped blocks are freed by the next level's forward-only kernel (the dealloc step
+        // below), their descendants by the levels after it
+        wave_append(drop_children, old_child - 1u, a.dlists[(level + 1) & 1], &cnt->dlist_count[level + 1], a.dlist_cap,
                     &cnt->frame_error, kErrQueue);
 
         // ---- liveness of the enqueued cells, for their children's pruned programs ----
@@ -263,7 +288,14 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         n_rev += (guard && td) ? 1 : 0;
         n_enq += enq ? 1 : 0;
     }
+    // ---- the dealloc cascade's step for the blocks dropped by the previous level and their
+    //      descendants (forward-only kernel; the reverse kernel of the level does not touch the
+    //      list, which the previous level's kernels finished) ----
+    uint64_t n_freed = 0;
+    if (!REV && level > 0) n_freed = dealloc_step(a, level, wave, nwaves);
+
     // ---- statistics: one atomic per wavefront ----
+    n_freed = wave_sum64(n_freed);
     n_cells = wave_sum64(n_cells);
     n_rev = wave_sum64(n_rev);
     n_par = wave_sum64(n_par);
@@ -279,6 +311,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         if (rev_ops) atomicAdd((unsigned long long *)&cnt->stats[kStatRevOps], (unsigned long long)rev_ops);
         if (fwd_ops) atomicAdd((unsigned long long *)&cnt->stats[kStatFwdOps], (unsigned long long)fwd_ops);
         if (n_pruned) atomicAdd((unsigned long long *)&cnt->stats[kStatPruned], (unsigned long long)n_pruned);
+        if (n_freed) atomicAdd((unsigned long long *)&cnt->stats[kStatDeallocs], (unsigned long long)n_freed);
     }
 }
 

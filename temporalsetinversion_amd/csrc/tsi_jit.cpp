@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -179,8 +180,18 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // only a decision (MIN/MAX: which operands receive [1, 1]; ABS: the sign class).  Each
     // value needed is saved once, when it is computed (row slot[k]); decisions are 2-bit codes,
     // 16 to a word, a word saved when its last code is set (row nslots + w).
-    std::vector<int> slot(n, -1), cidx(n, -1);
+    // A value of a one-register form with a literal (x + c, x - c, c - x, x * c, -x) whose
+    // register operand is saved anyway, or shared by several such values, is recomputed in the
+    // reverse pass from that operand (remat[k]: the operand) instead of being saved: the same
+    // binary32 operations give the same bits, and a coordinate shared by many primitives is
+    // then saved once.
+    std::vector<int> slot(n, -1), cidx(n, -1), remat(n, -1);
     uint32_t nslots = 0, ncodes = 0;
+    auto remat_form = [&](uint32_t k) {
+        const uint32_t f = fwd[k - 4].form;
+        return k >= 4 && (f == F_ADD_RL || f == F_SUB_RL || f == F_MUL_RL || f == F_NEG || f == F_SUB_LR);
+    };
+    auto reg_operand = [&](uint32_t k) { return side_of(tape[k]) == 2 ? rarg_of(tape[k]) : larg_of(tape[k]); };
     {
         std::vector<char> need(n, 0);
         for (uint32_t j = 4; j < n; ++j) {
@@ -192,6 +203,20 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             rev_needs(form, nx, ny);
             if (nx) need[side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])] = 1;
             if (ny) need[rarg_of(tape[j])] = 1;
+        }
+        std::vector<uint32_t> sharing(n, 0);
+        for (uint32_t k = 4; k < n; ++k)
+            if (need[k] && remat_form(k)) sharing[reg_operand(k)] += 1;
+        for (uint32_t k = 4; k < n; ++k) {   // ascending: an operand is final when reached
+            if (!need[k] || !remat_form(k)) continue;
+            // the operand (a < k) is decided already: saved, recomputed, or not needed so far
+            const uint32_t a = reg_operand(k);
+            const bool a_saved = need[a] && remat[a] < 0;
+            if (remat[a] < 0 && (a_saved || sharing[a] >= 2)) {
+                remat[k] = int(a);
+                need[k] = 0;
+                need[a] = 1;
+            }
         }
         for (uint32_t k = 0; k < n; ++k)
             if (need[k]) slot[k] = int(nslots++);
@@ -318,9 +343,17 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (!second) return nx ? int(side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])) : -1;
             return ny ? int(rarg_of(tape[j])) : -1;
         };
-        auto saved = [&](uint32_t j, int k) -> std::string {
+        std::function<std::string(uint32_t, int)> saved = [&](uint32_t j, int k) -> std::string {
             if (k < 0) return "ZI";
-            const std::string v = "I(sv" + std::to_string(k) + ".x, sv" + std::to_string(k) + ".y)";
+            std::string v;
+            if (remat[k] >= 0) {   // recomputed from its saved register operand
+                const std::string x = saved(j, remat[k]);
+                const bool lr = side_of(tape[k]) == 2;
+                v = "fixk(fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " + (lr ? "ZI, " + x : x + ", ZI") +
+                    ", " + lit(tape[k].literal) + "), K)";
+            } else {
+                v = "I(sv" + std::to_string(k) + ".x, sv" + std::to_string(k) + ".y)";
+            }
             if (pr && k >= 4 && region_of(uint32_t(k)) != region_of(j))
                 return "(" + bit(region_of(uint32_t(k))) + " ? " + v + " : ZI)";
             return v;
@@ -337,7 +370,8 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     continue;
                 }
                 for (bool second : {false, true}) {
-                    const int k = operand_of(jj, second);
+                    int k = operand_of(jj, second);
+                    if (k >= 0 && remat[k] >= 0) k = remat[k];
                     if (k >= 0 && !vdone[k]) o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
                     if (k >= 0) vdone[k] = 1;
                 }

@@ -155,32 +155,12 @@ __global__ void __launch_bounds__(256) k_prune(KArgs a, int level, int which) {
     }
 }
 
-// Dealloc cascade step (compute.wgsl:3533-3543): every node of a freed block that has
-// children passes its child block on; the block goes to the pending free list.
+// The dealloc cascade's steps after the last level (dealloc_step, tsi_eval.h).
 __global__ void __launch_bounds__(256) k_dealloc(KArgs a, int step) {
-    Counters *cnt = a.cnt;
-    const uint32_t count = min(cnt->dlist_count[step], a.dlist_cap);
-    if (count == 0) return;
-    const uint32_t *in = a.dlists[step & 1];
-    uint32_t *out = a.dlists[(step + 1) & 1];
-    const uint32_t lane = lane_id();
-    const uint32_t o = lane & 7u;
-    // wave index made visibly uniform: the grid-stride loop and everything inside it then
-    // compile as scalar control flow
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    uint64_t freed = 0;
-    for (uint32_t g = wave; g * 8u < count; g += nwaves) {
-        const uint32_t idx = g * 8u + (lane >> 3);
-        const bool active = idx < count;
-        const uint32_t b = active ? in[idx] : 0u;
-        const uint32_t m = active ? meta_of(a.st[b * 8u + o]) & kChildMask : 0u;
-        wave_append(m != 0, m - 1u, out, &cnt->dlist_count[step + 1], a.dlist_cap, &cnt->frame_error, kErrQueue);
-        wave_append(active && o == 0, b, a.pending_free, &cnt->pending, a.nblocks, &cnt->frame_error, kErrQueue);
-        freed += (active && o == 0) ? 1 : 0;
-    }
-    freed = wave_sum64(freed);
-    if (lane == 0 && freed) atomicAdd((unsigned long long *)&cnt->stats[kStatDeallocs], (unsigned long long)freed);
+    uint64_t freed = wave_sum64(dealloc_step(a, step, wave, nwaves));
+    if (lane_id() == 0 && freed) atomicAdd((unsigned long long *)&a.cnt->stats[kStatDeallocs], (unsigned long long)freed);
 }
 
 // Free-list merge: blocks handed out from the top of the stack are gone, freed blocks go on top.
@@ -469,7 +449,9 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
         }
     }
     if (e_eval1) hipEventRecord(e_eval1, s);
-    for (int c = 0; c <= fl.levels; ++c)
+    // levels 1..L-1 ran the cascade's steps 1..L-1 (eval_level); blocks dropped by the last
+    // level and the descendants of earlier drops need at most two more
+    for (int c = fl.levels; c <= fl.levels + 1; ++c)
         hipLaunchKernelGGL(k_dealloc, dim3(fl.grid_misc), dim3(256), 0, s, a, c);
     hipLaunchKernelGGL(k_frame_end, dim3(fl.grid_misc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_frame_commit, dim3(1), dim3(1), 0, s, a);
