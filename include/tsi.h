@@ -201,6 +201,9 @@ typedef struct {
     uint32_t prune_class;         /* 0 never pruned, 1 pruned up to ulp-level decisions, 2 exact */
     uint32_t engine;              /* 0 interpreter, 1 compiled straight-line kernels */
     uint32_t cache_hit;           /* compiled kernels came from the code-object cache */
+    uint32_t specialized;         /* the kernels assume every cell box inside [-5,5]^3 and every
+                                     t-window inside [-2, 3] (rebuilt without it otherwise) */
+    uint32_t fix_kept;            /* records whose fix is evaluated (the rest provably never clamp) */
     double compile_ms;            /* hiprtc compile (or cache lookup) wall time */
     char log[1024];               /* compiler log of a failed compile (truncated) */
 } tsi_tape_info;

@@ -84,7 +84,8 @@ def precompile(tape, prune=True) -> dict:
     rc = lib().tsi_precompile_tape(buf, len(t), flags, C.byref(info))
     if rc != 0:
         raise TsiError(rc, "tsi_precompile_tape\n" + info.log.decode(errors="replace"))
-    return {"compile_ms": info.compile_ms, "cache_hit": bool(info.cache_hit), "n": info.n}
+    return {"compile_ms": info.compile_ms, "cache_hit": bool(info.cache_hit), "n": info.n,
+            "specialized": bool(info.specialized), "fix_kept": info.fix_kept}
 
 
 class IR:

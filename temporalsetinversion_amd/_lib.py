@@ -66,7 +66,8 @@ class Stats(C.Structure):
 
 class TapeInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("stack", C.c_uint32), ("slots", C.c_uint32), ("prune_class", C.c_uint32),
-                ("engine", C.c_uint32), ("cache_hit", C.c_uint32), ("compile_ms", C.c_double),
+                ("engine", C.c_uint32), ("cache_hit", C.c_uint32), ("specialized", C.c_uint32),
+                ("fix_kept", C.c_uint32), ("compile_ms", C.c_double),
                 ("log", C.c_char * 1024)]
 
 

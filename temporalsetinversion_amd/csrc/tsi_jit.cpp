@@ -18,6 +18,7 @@
 #include <unistd.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -98,6 +99,103 @@ void rev_needs(uint32_t form, bool &x, bool &y) {
     }
 }
 
+// ---- where fix() can be left out ----
+// The binary32 interval rules of the forms below are inclusion-monotone: rounding to nearest is
+// monotone, so a rule applied to sub-intervals of its operands returns a sub-interval of what
+// it returns on the operands (for MUL the corner products of the smaller box lie between the
+// larger box's extreme corner products; min, max, sqrt, floor, negation and the clamp of fix
+// are monotone).  Evaluating the tape once on the host, with the device's operations, over a
+// domain that contains every cell box and t-window therefore bounds every value any cell can
+// compute.  Where a record's value (before fix) stays inside [-1e6, 1e6] on the domain it is
+// never NaN or out of range in a cell, and fix leaves it unchanged.  sin / cos get the bound
+// [-1.001, 1.001] (their deterministic routines stay within an ulp of [-1, 1]); every other form,
+// and everything computed from it, gets none.  Bounds include 0: a record of a region that the
+// pruned code skipped reads [+0, +0].
+namespace {
+
+constexpr float kBig = 1000000.0f;   // fix's bound (compute.wgsl:2534)
+
+struct HBound { float lo, hi; bool ok; };
+
+float hmin(float a, float b) { return __builtin_fminimum_numf(a, b); }
+float hmax(float a, float b) { return __builtin_fmaximum_numf(a, b); }
+
+HBound hb(float lo, float hi) {
+    const bool ok = lo == lo && hi == hi && lo >= -kBig && hi <= kBig;
+    return HBound{lo, hi, ok};
+}
+
+// the device rule of `form` (tsi_rules.h fwd_form_t, tsi_interval.h), before fix
+HBound bound_rule(uint32_t form, HBound l, HBound r, float c) {
+    const HBound none{0.0f, 0.0f, false};
+    auto mul = [](float alo, float ahi, float blo, float bhi) {
+        const float x = alo * blo, y = alo * bhi, z = ahi * blo, w = ahi * bhi;
+        return hb(hmin(hmin(x, y), hmin(z, w)), hmax(hmax(x, y), hmax(z, w)));
+    };
+    switch (form) {
+    case F_ADD_RR: return l.ok && r.ok ? hb(l.lo + r.lo, l.hi + r.hi) : none;
+    case F_ADD_RL: return l.ok ? hb(l.lo + c, l.hi + c) : none;
+    case F_SUB_RR: return l.ok && r.ok ? hb(l.lo - r.hi, l.hi - r.lo) : none;
+    case F_SUB_RL: return l.ok ? hb(l.lo - c, l.hi - c) : none;
+    case F_SUB_LR: return r.ok ? hb(c - r.hi, c - r.lo) : none;
+    case F_MUL_RR: return l.ok && r.ok ? mul(l.lo, l.hi, r.lo, r.hi) : none;
+    case F_MUL_RL: {
+        if (!l.ok) return none;
+        const float x = l.lo * c, y = l.hi * c;
+        return c >= 0.0f ? hb(x, y) : hb(y, x);
+    }
+    case F_DIV_RL: {
+        if (!l.ok || !(c < 0.0f || c > 0.0f)) return none;
+        const float x = l.lo / c, z = l.hi / c;
+        return hb(hmin(x, z), hmax(x, z));
+    }
+    case F_SQR: {
+        if (!l.ok) return none;
+        const float x = l.lo * l.lo, y = l.hi * l.hi;
+        return hb(l.lo <= 0.0f && 0.0f <= l.hi ? 0.0f : hmin(x, y), hmax(x, y));
+    }
+    case F_SQRT: return l.ok ? hb(std::sqrt(hmax(l.lo, 0.0f)), std::sqrt(hmax(l.hi, 0.0f))) : none;
+    case F_MIN_RR: return l.ok && r.ok ? hb(hmin(l.lo, r.lo), hmin(l.hi, r.hi)) : none;
+    case F_MIN_RL: return l.ok ? hb(hmin(l.lo, c), hmin(l.hi, c)) : none;
+    case F_MAX_RR: return l.ok && r.ok ? hb(hmax(l.lo, r.lo), hmax(l.hi, r.hi)) : none;
+    case F_MAX_RL: return l.ok ? hb(hmax(l.lo, c), hmax(l.hi, c)) : none;
+    case F_NEG: return l.ok ? hb(-l.hi, -l.lo) : none;
+    case F_ABS: {
+        if (!l.ok) return none;
+        if (l.lo <= 0.0f && 0.0f <= l.hi) return hb(0.0f, hmax(-l.lo, l.hi));
+        return l.hi < 0.0f ? hb(-l.hi, -l.lo) : l;
+    }
+    case F_RET: return l;
+    case F_FLOOR: return l.ok ? hb(std::floor(l.lo), std::floor(l.hi)) : none;
+    case F_SIN: case F_COS: return l.ok ? hb(-1.001f, 1.001f) : none;
+    case F_NOP_RR: return hb(c, c);
+    case F_ZERO: return hb(0.0f, 0.0f);
+    default: return none;
+    }
+}
+
+}  // namespace
+
+std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
+                           const JitDomain &d) {
+    std::vector<HBound> b(n);
+    std::vector<char> free(n, 0);
+    for (uint32_t k = 0; k < 3; ++k) b[k] = hb(hmin(d.box[k], 0.0f), hmax(d.box[3 + k], 0.0f));
+    b[3] = hb(hmin(d.t0, 0.0f), hmax(d.t1, 0.0f));
+    for (uint32_t j = 4; j < n; ++j) {
+        const tsi_gpu_instruction &g = tape[j];
+        const uint32_t side = side_of(g);
+        const HBound zero{0.0f, 0.0f, true};
+        const HBound l = (side == 0 || side == 1) ? b[larg_of(g)] : zero;
+        const HBound r = (side == 0 || side == 2) ? b[rarg_of(g)] : zero;
+        const HBound v = side == 3 ? hb(0.0f, 0.0f) : bound_rule(fwd[j - 4].form, l, r, g.literal);
+        free[j] = v.ok ? 1 : 0;
+        // after fix (the identity where free), with 0 (a skipped region's value)
+        b[j] = v.ok ? hb(hmin(v.lo, 0.0f), hmax(v.hi, 0.0f)) : HBound{0.0f, 0.0f, false};
+    }
+    return free;
+}
+
 JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L) {
     JitLayout y;
     y.group = region_records();
@@ -110,7 +208,8 @@ JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L) {
 }
 
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
-                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune) {
+                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
+                       const JitDomain *domain) {
     std::string o;
     o.reserve(size_t(n) * 500u);
     const JitLayout Y = jit_layout(fwd, L);
@@ -120,6 +219,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     std::vector<int> mmx(n, -1);
     for (uint32_t j = 4, m = 0; j < n; ++j)
         if (is_minmax_form(fwd[j - 4].form)) mmx[j] = int(m++);
+    // records whose fix is the identity on the domain the kernels are specialised for
+    const std::vector<char> ff = domain ? fix_free(tape, n, fwd, *domain) : std::vector<char>(n, 0);
+    auto fixed = [&](uint32_t j, const std::string &e) { return ff[j] ? e : "fixk(" + e + ", K)"; };
     // the variant being emitted: pruned (region mask, parent modes, choices) or full
     bool pr = prune;
     auto bit = [&](uint32_t r) {   // uniform test of region r's bit in the wave's region mask
@@ -270,8 +372,8 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
             const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
             o += "        const uint32_t md" + J + " = " + mode_of(j) + ";\n";
-            o += "        " + V(j) + " = fixk(sel(md" + J + " == 1u, " + l + ", sel(md" + J + " == 2u, " + rhs +
-                 ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "))), K);\n";
+            o += "        " + V(j) + " = " + fixed(j, "sel(md" + J + " == 1u, " + l + ", sel(md" + J + " == 2u, " + rhs +
+                 ", fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + ")))") + ";\n";
             const int m = mmx[j];
             const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
             o += "        const uint32_t ch" + J + " = md" + J + " != 3u ? md" + J + " : " +
@@ -281,12 +383,12 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const bool rl = f.form == F_MIN_RL || f.form == F_MAX_RL;
             const bool is_min = f.form == F_MIN_RR || f.form == F_MIN_RL;
             const std::string rhs = rl ? "I_FROM(" + c + ")" : rr;
-            o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
+            o += "        " + V(j) + " = " + fixed(j, "fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + ")") + ";\n";
             const int m = mmx[j];
             const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
             o += "        cw" + w + " |= " + (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ") << " + sh + ";\n";
         } else {
-            o += "        " + V(j) + " = fixk(fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + "), K);\n";
+            o += "        " + V(j) + " = " + fixed(j, "fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + ")") + ";\n";
         }
         if (slot[j] >= 0)
             o += "        if (PUSH) " + P(uint32_t(slot[j])) + " = make_float2(" + V(j) + ".lo, " + V(j) + ".hi);\n";
@@ -349,8 +451,8 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (remat[k] >= 0) {   // recomputed from its saved register operand
                 const std::string x = saved(j, remat[k]);
                 const bool lr = side_of(tape[k]) == 2;
-                v = "fixk(fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " + (lr ? "ZI, " + x : x + ", ZI") +
-                    ", " + lit(tape[k].literal) + "), K)";
+                v = fixed(uint32_t(k), "fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " +
+                                           (lr ? "ZI, " + x : x + ", ZI") + ", " + lit(tape[k].literal) + ")");
             } else {
                 v = "I(sv" + std::to_string(k) + ".x, sv" + std::to_string(k) + ".y)";
             }
@@ -444,6 +546,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                         const std::string d = "d" + J + "." + dx;
                         if (f.form == F_MUL_RR && (dx[0] == 'a' ? b : a) >= 4u) return d;
                         if (cidx[j] >= 0) return d;   // 0, 1 or -1: fix leaves them
+                        // on a domain: MUL_RR's derivatives are argument values inside it, and
+                        // SQR's is 2x with |x| <= 1000 where x^2 stays inside [-1e6, 1e6]
+                        if (domain && (f.form == F_MUL_RR || (f.form == F_SQR && ff[j]))) return d;
                         return "fixk(" + d + ", K)";
                     };
                     // the first contribution to a sensitivity is not added to [+0, +0]: that

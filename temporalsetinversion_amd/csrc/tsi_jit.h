@@ -16,12 +16,20 @@ struct JitLayout {
 };
 JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L);
 
+// The domain a tape's kernels may be specialised for: every cell box inside `box` (lo xyz, hi
+// xyz), every t-window inside [t0, t1].  fix_free marks the records whose value no cell in it can
+// take outside [-1e6, 1e6] (fix then leaves the value unchanged and is not emitted).
+struct JitDomain { float box[6]; float t0, t1; };
+std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
+                           const JitDomain &d);
+
 // HIP source of the level kernel for one tape: tsi_tape_eval_fwd (forward-only frontier list)
 // or tsi_tape_eval_rev (forward with saved operands, then reverse).  prune: regions dead for
 // all of a wavefront's parents are skipped and decided MIN/MAX become copies (liveness records
 // written for enqueued cells).
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
-                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune);
+                       const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
+                       const JitDomain *domain = nullptr);
 
 // Compiles `src` for gfx950 with hiprtc (the library's kernel headers are embedded), or finds
 // the code object in this process's cache or the on-disk cache ($TSI_JIT_CACHE, default

@@ -87,6 +87,11 @@ struct tsi_ctx {
     FrameSlot fslot[kFrameSlots];
     TraceSlot tslot[kTraceSlots];
     uint64_t f_issued = 0, f_done = 0, t_issued = 0, t_done = 0;
+    // the domain the loaded tape kernels are specialised for (they leave out fix where it is the
+    // identity there), the root box of the last reset and the t-windows seen since
+    bool jit_specialized = false;
+    float root_box[6] = {-5.0f, -5.0f, -5.0f, 5.0f, 5.0f, 5.0f};
+    float t_seen_lo = 1.0f, t_seen_hi = 0.0f;   // empty
 };
 
 namespace {
@@ -225,16 +230,21 @@ void release_jit(tsi_ctx *c) {
 }
 
 // Compiles (or finds cached) the tape's two level kernels and loads them on this device.
+// The root box of the reference (main.rs) and a t-range around the frame loop's [0, 1]: the tape
+// kernels are compiled for it, and recompiled without that assumption if a frame leaves it.
+const JitDomain kDefaultDomain{{-5.0f, -5.0f, -5.0f, 5.0f, 5.0f, 5.0f}, -2.0f, 3.0f};
+
 int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
-             const std::vector<RevIns> &rev, const Lowered &L) {
+             const std::vector<RevIns> &rev, const Lowered &L, const JitDomain *domain) {
     release_jit(c);
+    c->jit_specialized = false;
     std::string src[2], log[2];
     std::vector<char> code[2];
     bool hit[2] = {false, false};
     double ms[2] = {0.0, 0.0};
     int rc[2] = {TSI_OK, TSI_OK};
     try {
-        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, c->prune);
+        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, c->prune, domain);
         // the two kernels compile in parallel
         std::thread t([&] { rc[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
         rc[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
@@ -260,6 +270,13 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
             release_jit(c);
             return TSI_E_HIP;
         }
+    }
+    c->jit_specialized = domain != nullptr;
+    c->info.specialized = domain ? 1u : 0u;
+    c->info.fix_kept = n - 4u;
+    if (domain) {
+        const std::vector<char> ff = fix_free(tape, n, fwd, *domain);
+        for (uint32_t j = 4; j < n; ++j) c->info.fix_kept -= ff[j] ? 1u : 0u;
     }
     return TSI_OK;
 }
@@ -451,7 +468,7 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     if (!(c->p.flags & TSI_FLAG_INTERPRET)) {
         fwd.resize(L.nfwd);
         rev.resize(L.nrev);
-        if ((rc = load_jit(c, tape, n, fwd, rev, L))) {
+        if ((rc = load_jit(c, tape, n, fwd, rev, L, &kDefaultDomain))) {
             c->have_tape = false;
             return rc;
         }
@@ -472,6 +489,9 @@ int tsi_reset(tsi_ctx *c, const float root_box[6]) {
     KArgs a = make_args(c);
     if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
     HIP_OK(hipStreamSynchronize(c->stream));
+    std::memcpy(c->root_box, root_box, sizeof c->root_box);
+    c->t_seen_lo = 1.0f;
+    c->t_seen_hi = 0.0f;
     return TSI_OK;
 }
 
@@ -487,10 +507,48 @@ int tsi_enqueue_root(tsi_ctx *c) {
     return TSI_OK;
 }
 
+namespace {
+
+// Every cell box lies in the root box and every t-window in the span of the frames' times since
+// the reset (each swept window reaching swept_window below its frame's time): when these leave
+// the domain the kernels were specialised for, the kernels are rebuilt without the assumption.
+int keep_domain(tsi_ctx *c) {
+    const float t = c->u.time;
+    const float lo = c->p.swept_volume ? t - c->p.swept_window : t;
+    if (c->t_seen_lo > c->t_seen_hi) {
+        c->t_seen_lo = lo;
+        c->t_seen_hi = t;
+    } else {
+        c->t_seen_lo = std::fmin(c->t_seen_lo, lo);
+        c->t_seen_hi = std::fmax(c->t_seen_hi, t);
+    }
+    if (!c->jit_specialized) return TSI_OK;
+    const JitDomain &d = kDefaultDomain;
+    bool inside = c->t_seen_lo >= d.t0 && c->t_seen_hi <= d.t1;
+    for (int k = 0; k < 3; ++k) inside = inside && c->root_box[k] >= d.box[k] && c->root_box[3 + k] <= d.box[3 + k];
+    if (inside) return TSI_OK;
+    HIP_OK(hipStreamSynchronize(c->stream));   // frames in flight use the loaded kernels
+    HIP_OK(hipStreamSynchronize(c->tstream));
+    std::vector<FwdIns> fwd;
+    std::vector<RevIns> rev;
+    Lowered L;
+    int rc = lower(c->tape.data(), uint32_t(c->tape.size()), fwd, rev, L);
+    if (rc) return rc;
+    fwd.resize(L.nfwd);
+    rev.resize(L.nrev);
+    return load_jit(c, c->tape.data(), uint32_t(c->tape.size()), fwd, rev, L, nullptr);
+}
+
+}  // namespace
+
 int tsi_set_inversion_async(tsi_ctx *c) {
     if (!c) return TSI_E_ARG;
     if (!c->have_tape) return TSI_E_NO_TAPE;
     if (c->f_issued - c->f_done >= tsi_ctx::kFrameSlots) return TSI_E_PENDING;
+    if (c->eval_fn[0]) {
+        const int rc = keep_domain(c);
+        if (rc) return rc;
+    }
     tsi_ctx::FrameSlot &f = c->fslot[c->f_issued % tsi_ctx::kFrameSlots];
     KArgs a = make_args(c);
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
@@ -719,7 +777,12 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t fl
             bool hit[2] = {false, false};
             double ms[2] = {0.0, 0.0};
             int r[2] = {TSI_OK, TSI_OK};
-            for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, prune_wanted(flags, L));
+            for (int k = 0; k < 2; ++k)
+                src[k] = jit_source(tape, n, fwd, rev, L, k == 1, prune_wanted(flags, L), &kDefaultDomain);
+            info.specialized = 1;
+            info.fix_kept = n - 4u;
+            const std::vector<char> ff = fix_free(tape, n, fwd, kDefaultDomain);
+            for (uint32_t j = 4; j < n; ++j) info.fix_kept -= ff[j] ? 1u : 0u;
             std::thread t([&] { r[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
             r[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
             t.join();

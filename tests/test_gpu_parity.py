@@ -210,3 +210,24 @@ def test_frames_in_flight_match_blocking_frames():
             assert a[key] == b[key], key
     assert_same_nodes(got_nodes, want_nodes, "after the last frame")
     assert np.array_equal(img.view(np.uint32), want_img[-1].view(np.uint32))
+
+
+def test_frames_outside_the_specialised_domain():
+    """The tape kernels leave out fix where it is the identity for cells in [-5,5]^3 and
+    t-windows in [-2, 3]; a frame at t = 4 leaves that domain, the kernels are rebuilt with every
+    fix, and results stay bit-identical to the oracle before and after."""
+    tape = tape_of("sphere")   # small: its kernels without the assumption compile in seconds
+    times = [0.0, 1 / 60, 4.0, 4.0 + 1 / 60]
+    with T.State(tape, localsubdiv=4, voxel_capacity=1 << 16) as st:
+        assert st.tape_info()["specialized"]
+    seen = []
+    for k, g, o, gs, os_, _, _ in run_pair(tape, times, localsubdiv=5):
+        print(f"frame={k} t={times[k]} nodes={len(o)}", flush=True)
+        assert_same_nodes(g, o, f"frame {k} (t = {times[k]})")
+        seen.append(k)
+    assert seen == [0, 1, 2, 3]
+    with T.State(tape, localsubdiv=4, voxel_capacity=1 << 16) as st:
+        st.reset()
+        st.update(4.0)
+        st.set_inversion()
+        assert not st.tape_info()["specialized"]
