@@ -183,10 +183,10 @@ def main():
                     help="trace each frame synchronously instead of overlapping it with the next set inversion")
     ap.add_argument("--prune", default="auto", choices=["auto", "on", "off", "exact"],
                     help="liveness-pruned evaluation: on, off (full tape per cell), exact (only for "
-                         "tapes whose pruned results are provably identical), auto (default: on for "
-                         "swept-volume workloads, where every cell's t-window lies inside its "
-                         "parent's, off otherwise, where most reverse-list cells cannot use their "
-                         "parent's record and the bookkeeping costs more than it saves)")
+                         "tapes whose pruned results are provably identical), auto (default): off for "
+                         "these workloads, where it does not pay (capsules: the reverse-list cells "
+                         "cannot use their parents' records; csg: the swept windows keep ~80 %% of "
+                         "the records live)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="instead of the benchmark: copy a buffer of this many MiB device-to-device 5 times "
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -226,7 +226,7 @@ def main():
     import temporalsetinversion_amd as T
 
     tape = T.tape_from_ll(ll)
-    prune = args.prune if args.prune != "auto" else ("on" if w["swept_volume"] else "off")
+    prune = args.prune if args.prune != "auto" else "off"
     st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
                  point_sample=w["point_sample"], swept_volume=w["swept_volume"],
                  lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27,

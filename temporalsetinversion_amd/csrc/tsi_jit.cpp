@@ -559,6 +559,16 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                         return live[k] ? "ADD_II(" + prod + ", " + S(k) + ")" : prod;
                     };
                     auto old = [&](uint32_t k) { return live[k] ? S(k) : std::string("ZI"); };
+                    // Short-circuit (compute.wgsl:3802-3805): a single-use operand of a record
+                    // whose sensitivity is [0, 0] gets [0, 0].  After fix, the derivative of
+                    // these forms always has a finite bound (it is ordered, or a decision code),
+                    // so the product with a zero sensitivity is already +-0: the select would
+                    // only fix the sign of a zero, and it is left out (the work count stays).
+                    const bool zero_product = is_code_form(f.form) || f.form == F_ADD_RR || f.form == F_ADD_RL ||
+                        f.form == F_SUB_RR || f.form == F_SUB_RL || f.form == F_SUB_LR || f.form == F_NEG ||
+                        f.form == F_RET || f.form == F_MUL_RL || f.form == F_DIV_RL || f.form == F_MUL_RR ||
+                        f.form == F_SQR || f.form == F_SQRT || f.form == F_DIV_RR || f.form == F_DIV_LR ||
+                        f.form == F_MOD_RL;
                     if (side == 1 || side == 2) {
                         const uint32_t k = side == 1 ? a : b;
                         const bool u1 = side == 1 ? (rv.flags & R_U1_A) != 0 : (rv.flags & R_U1_B) != 0;
@@ -567,7 +577,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                         if (u1) {
                             o += "        const bool z" + J + " = " + sj0 + ";\n";
                             o += "        ops += z" + J + " ? 0u : " + cost + ";\n";
-                            val = "sel(z" + J + ", ZI, " + val + ")";
+                            if (!zero_product) val = "sel(z" + J + ", ZI, " + val + ")";
                         } else {
                             static_ops += rv.cost;
                         }
@@ -583,8 +593,10 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                         if (u1) {
                             o += "        const bool z" + J + " = " + sj0 + ";\n";
                             o += "        ops += z" + J + " ? 0u : " + cost + ";\n";
-                            va = "sel(z" + J + ", ZI, " + va + ")";
-                            vb = "sel(z" + J + ", ZI, " + vb + ")";
+                            if (!zero_product) {
+                                va = "sel(z" + J + ", ZI, " + va + ")";
+                                vb = "sel(z" + J + ", ZI, " + vb + ")";
+                            }
                         } else {
                             static_ops += rv.cost;
                         }
