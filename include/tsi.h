@@ -73,8 +73,15 @@ typedef struct {
     uint32_t prune_live_max;      /* compiled engine: a wavefront runs its pruned code when at
                                      most this percentage of the tape's regions is live for it
                                      (0 = 75; 100 = whenever its parents' records apply) */
-    uint32_t reserved;
+    uint32_t shading;             /* TSI_SHADE_* (0: the reference's default colouring) */
 } tsi_params;
+
+/* compute.wgsl:3503-3504 normal_viz / fast_normals as runtime values.  TSI_SHADE_NORMALS: hits
+ * show normalize(normal) * 0.5 + 0.5, misses (10, 10, 10, 0) before the 0.45 gamma, no octree
+ * darkening (4265, 4333-4377, 4510-4514).  With TSI_SHADE_GRADIENT as well (fast_normals = false)
+ * the normal is the voxel's reverse-pass gradient (dx, dy, dz) and every cell takes the reverse
+ * pass (3780); without it, it is finite_diff_normal's value for a tape scene, (1, 1, 1). */
+enum { TSI_SHADE_NORMALS = 1u, TSI_SHADE_GRADIENT = 2u };
 
 enum {
     /* Evaluate the full tape in every cell (the reference's behaviour).  Without it, a cell

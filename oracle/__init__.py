@@ -29,7 +29,7 @@ class Iv(C.Structure):
 class Params(C.Structure):
     _fields_ = [("localsubdiv", C.c_int32), ("delta", C.c_float), ("point_sample", C.c_int32),
                 ("swept_volume", C.c_int32), ("lipschitz_corrected", C.c_int32), ("fastenabled", C.c_int32),
-                ("fast_multiplier", C.c_float), ("swept_window", C.c_float)]
+                ("fast_multiplier", C.c_float), ("swept_window", C.c_float), ("shading", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -154,10 +154,10 @@ class Frame:
 
     def __init__(self, tape: np.ndarray, localsubdiv=6, delta=0.01, point_sample=True, swept_volume=False,
                  lipschitz_corrected=False, fastenabled=True, fast_multiplier=1.0, swept_window=1.0 / 60.0,
-                 voxel_capacity=1 << 22, root_box=(-5.0, -5.0, -5.0, 5.0, 5.0, 5.0)):
+                 voxel_capacity=1 << 22, root_box=(-5.0, -5.0, -5.0, 5.0, 5.0, 5.0), shading=0):
         self._tape, p = _tape_ptr(tape)
         self.params = Params(localsubdiv, delta, int(point_sample), int(swept_volume), int(lipschitz_corrected),
-                             int(fastenabled), fast_multiplier, np.float32(swept_window))
+                             int(fastenabled), fast_multiplier, np.float32(swept_window), int(shading))
         self._h = lib().orc_create(C.byref(self.params), p, len(self._tape), voxel_capacity)
         if not self._h:
             raise MemoryError("orc_create failed")

@@ -224,7 +224,8 @@ static void process_parent(orc_state *s, uint32_t P, float time, float epsilon, 
         if (!(new_voxel || cv->f >= 0.0f || cv->child_count == 0u))      /* 3629 */
             continue;
         iv args[4] = {cx, cy, cz, I(wmin(ct.lo, ct.hi), wmax(ct.lo, ct.hi))};
-        int take_derivative = ct.lo != ct.hi;                             /* 3780 */
+        int take_derivative = ct.lo != ct.hi ||                           /* 3780 */
+                              ((p->shading & ORC_SHADE_NORMALS) && (p->shading & ORC_SHADE_GRADIENT));
         iv field = orc_eval_tape(s->tape, s->n, args, take_derivative, s->vals, s->sens,
                                  st ? &st->fwd_ops : NULL, st ? &st->rev_ops : NULL);
         if (st) { st->cell_evals++; if (take_derivative) st->rev_evals++; }
@@ -342,9 +343,13 @@ static inline float mylerp(float a, float b, float t)        /* 4116: mix(a, b, 
     return a * (1.0f - tt) + b * tt;
 }
 
-/* compute.wgsl:4189-4403 (non-cinematic, non-normal_viz colour). */
+/* normalize (WGSL), as v / length(v) */
+static inline v3 vnormalize(v3 v) { float l = vlength(v); return V(v.x / l, v.y / l, v.z / l); }
+
+/* compute.wgsl:4189-4403 (non-cinematic colour; normal_viz with s->p.shading). */
 static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, float *dist, uint64_t *visits)
 {
+    const int normal_viz = (s->p.shading & ORC_SHADE_NORMALS) != 0;
     const orc_voxel *vox = s->vox;
     uint32_t stack[16];
     uint32_t sp = 0;
@@ -390,7 +395,7 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
         float mkx = 0.0f, mky = 0.0f, mkz = 0.0f;
         v3 old = o2;
         for (int i = 0; i < 32; i++) {
-            if (max_component(vsub(p1, p0)) < 1.0f) c = vscale(c, DARKEN);             /* 4265 */
+            if (!normal_viz && max_component(vsub(p1, p0)) < 1.0f) c = vscale(c, DARKEN);   /* 4265 */
             float t = ((vlength(V(mkx * (sd.x - dd.x), mky * (sd.y - dd.y), mkz * (sd.z - dd.z))) + hit.y) + TR_EPS) + sms;
             o2 = vadd(old, vscale(rd, t));
             sms = 0.0f;
@@ -422,6 +427,13 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
                         break;
                     } else {
                         intersected = 1;
+                        if (normal_viz) {                                     /* 4333-4377 */
+                            v3 n = (s->p.shading & ORC_SHADE_GRADIENT) ? V(vox[child].dx, vox[child].dy, vox[child].dz)
+                                                                         : V(1.0f, 1.0f, 1.0f);
+                            v3 nv = vnormalize(V(n.x * 1.0f, n.y * 1.0f, n.z * 1.0f));
+                            c = V(nv.x * 0.5f + 0.5f, nv.y * 0.5f + 0.5f, nv.z * 0.5f + 0.5f);
+                            break;
+                        }
                         float ts = time - vox[child].t;
                         float tt = ts * 60.0f;
                         c = V(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
@@ -469,9 +481,15 @@ int orc_trace(const orc_state *s, const float view[16], uint32_t W, uint32_t H, 
             v3 c;
             float dist;
             trace_ray(s, eye, g, time, &c, &dist, node_visits);
-            /* get_color_single returns vec4(c, 1) on hit and on the first-bounce miss;
+            /* get_color_single returns vec4(c, 1) on hit and on the first-bounce miss, and
+             * vec4(10, 10, 10, 0) on a miss with normal_viz (4510-4514);
              * get_color_supersample: c = (0 + c) / 1, then pow(c, 0.45) per channel. */
-            float r = (0.0f + c.x) / 1.0f, gg = (0.0f + c.y) / 1.0f, b = (0.0f + c.z) / 1.0f, a = (0.0f + 1.0f) / 1.0f;
+            float ca = 1.0f;
+            if ((s->p.shading & ORC_SHADE_NORMALS) && !(dist >= 0.0f)) {
+                c = V(10.0f, 10.0f, 10.0f);
+                ca = 0.0f;
+            }
+            float r = (0.0f + c.x) / 1.0f, gg = (0.0f + c.y) / 1.0f, b = (0.0f + c.z) / 1.0f, a = (0.0f + ca) / 1.0f;
             size_t pi = (size_t)y * W + x;
             if (rgba) {
                 rgba[4 * pi + 0] = orc_det_pow045(r);

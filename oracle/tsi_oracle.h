@@ -64,7 +64,14 @@ typedef struct {
     int32_t fastenabled;
     float fast_multiplier;
     float swept_window;   /* 1/60 in compute.wgsl:3605 */
+    int32_t shading;      /* ORC_SHADE_* (compute.wgsl:3503-3504 normal_viz, fast_normals) */
 } orc_params;
+
+/* normal_viz (compute.wgsl:3503): hits show normalize(normal) * 0.5 + 0.5, misses (10, 10, 10, 0),
+ * no octree darkening.  With ORC_SHADE_GRADIENT (fast_normals = false, 3504) the normal is the
+ * voxel's reverse-pass gradient and every cell takes the reverse pass (3780); otherwise it is
+ * finite_diff_normal's value for a tape scene, (1, 1, 1) (4143-4184 `default`). */
+enum { ORC_SHADE_NORMALS = 1, ORC_SHADE_GRADIENT = 2 };
 
 typedef struct {
     uint64_t cell_evals;     /* lanes that passed the evaluation guard (3629) */

@@ -154,7 +154,7 @@ class State:
                  point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
                  fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
                  voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0,
-                 engine: str = "compiled", prune_live_max: int = 0):
+                 engine: str = "compiled", prune_live_max: int = 0, shading: int = 0):
         if isinstance(tape, IR):
             tape = tape.gpu_instructions
         elif isinstance(tape, (str, bytes)):
@@ -185,6 +185,10 @@ class State:
             p.flags |= _lib.TSI_FLAG_INTERPRET
         p.prune_capacity = prune_capacity
         p.prune_live_max = prune_live_max
+        # shading: 0 = the reference's colouring; TSI_SHADE_NORMALS (1) = normal_viz with
+        # finite_diff_normal's (1, 1, 1) for a tape; | TSI_SHADE_GRADIENT (2) = normal_viz with
+        # the voxels' reverse-pass gradients (every cell takes the reverse pass)
+        p.shading = shading
         self.params = p
         self._ctx = C.c_void_p()
         check(lib().tsi_create(C.byref(p), C.byref(self._ctx)), "tsi_create")

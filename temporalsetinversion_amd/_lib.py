@@ -24,6 +24,8 @@ ERRORS = {
 TSI_FLAG_NO_PRUNE = 1
 TSI_FLAG_PRUNE_EXACT_ONLY = 2
 TSI_FLAG_INTERPRET = 4
+TSI_SHADE_NORMALS = 1
+TSI_SHADE_GRADIENT = 2
 
 
 class TsiError(RuntimeError):
@@ -50,7 +52,7 @@ class Params(C.Structure):
                 ("swept_volume", C.c_int32), ("lipschitz_corrected", C.c_int32), ("fastenabled", C.c_int32),
                 ("fast_multiplier", C.c_float), ("swept_window", C.c_float), ("voxel_capacity", C.c_uint64),
                 ("queue_capacity", C.c_uint64), ("flags", C.c_uint32), ("prune_capacity", C.c_uint32),
-                ("prune_live_max", C.c_uint32), ("reserved", C.c_uint32)]
+                ("prune_live_max", C.c_uint32), ("shading", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -151,7 +151,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         }
         const bool guard = ok && (isnew || cst.x >= 0.0f || (cmeta & kChildMask) == 0);
         const Iv targ = I_FROM_UNORDERED(ct.lo, ct.hi);
-        const bool td = ct.lo != ct.hi;
+        const bool td = ct.lo != ct.hi || a.force_deriv != 0;   // take_derivative (3780)
 
         // ---- program: the group's pruned program when every evaluated child lies inside its
         //      parent (box and t-window of the evaluation that produced the liveness record),
@@ -259,8 +259,8 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         if (ok && (isnew || (REV && guard && td))) a.grad[C] = gradv;
 
         // ---- compaction of the outputs ----
-        const bool next_fwd_only =
-            !a.swept && ((cmeta & kChildMask) == 0 || (cmeta & kFastBit) || width / 2.0f <= eps);
+        const bool next_fwd_only = !a.swept && !a.force_deriv &&
+                                   ((cmeta & kChildMask) == 0 || (cmeta & kFastBit) || width / 2.0f <= eps);
         const uint32_t at_f = wave_append(enq && next_fwd_only, C, next_f, &cnt->list_count[level + 1][0],
                                           a.list_cap, &cnt->frame_error, kErrQueue);
         const uint32_t at_r = wave_append(enq && !next_fwd_only, C, next_r, &cnt->list_count[level + 1][1],

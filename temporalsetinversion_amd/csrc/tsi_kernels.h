@@ -60,6 +60,8 @@ struct KArgs {
     uint32_t nblocks, list_cap, dlist_cap;
     float epsilon, delta, fast_multiplier, swept_window;
     int32_t swept, point_sample, lipschitz, fastenabled;
+    // normal_viz with fast_normals off (compute.wgsl:3780): every cell takes the reverse pass
+    int32_t force_deriv;
     // pruned programs (per-group dense programs specialised on the parents' liveness)
     int32_t prune;
     uint32_t *prune_rec[2][2];  // [level parity][forward-only / reverse], see kRecHead
@@ -80,6 +82,8 @@ struct TraceArgs {
     float4 *rgba;
     float *depth;
     unsigned long long *visits;
+    uint32_t shading;           // TSI_SHADE_* (normal_viz, gradient normals)
+    const float4 *grad;         // the nodes' dx, dy, dz (gradient normals)
 };
 
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tsi.h"
 #include "tsi_eval.h"
 #include "tsi_interp.h"
 #include "tsi_launch.h"
@@ -52,7 +53,8 @@ __global__ void k_frame_begin(KArgs a) {
             const float4 lo = a.blo[0], hi = a.bhi[0];
             const float dx = (hi.x - lo.x) / 2.0f, dy = (hi.y - lo.y) / 2.0f, dz = (hi.z - lo.z) / 2.0f;
             const float w = wmax(dx, wmax(dy, dz));
-            const bool fwd_only = !a.swept && ((meta & kChildMask) == 0 || (meta & kFastBit) || w <= a.epsilon);
+            const bool fwd_only = !a.swept && !a.force_deriv &&
+                                  ((meta & kChildMask) == 0 || (meta & kFastBit) || w <= a.epsilon);
             const int which = fwd_only ? 0 : 1;
             a.lists[0][which][0] = 0u;
             c->list_count[0][which] = 1;
@@ -232,8 +234,15 @@ TSI_DI float mylerp(float a, float b, float t) {   // compute.wgsl:4116
 }
 
 // compute.wgsl:4189-4403 for the non-cinematic, non-normal_viz colour; the stack lives in LDS.
+// normalize (WGSL), as v / length(v)
+TSI_DI V3 vnormalize(V3 v) {
+    const float l = vlength(v);
+    return v3(v.x / l, v.y / l, v.z / l);
+}
+
 TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V3 &col, float &dist,
-                      uint32_t &visits) {
+                      uint32_t &visits, uint32_t shading, const float4 *grad) {
+    const bool normal_viz = (shading & TSI_SHADE_NORMALS) != 0;
     constexpr uint32_t FAIL = 0xFFFFFFFFu;
     uint32_t sp = 0;
     V3 c = v3(1.0f, 1.0f, 1.0f);
@@ -298,7 +307,7 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
         const bool qown = qnode == region;
         const float hx = (q1.x - q0.x) / 2.0f, hy = (q1.y - q0.y) / 2.0f, hz = (q1.z - q0.z) / 2.0f;
         for (int i = 0; i < 32; ++i) {
-            if (wmax(p1.x - p0.x, wmax(p1.y - p0.y, p1.z - p0.z)) < 1.0f) {
+            if (!normal_viz && wmax(p1.x - p0.x, wmax(p1.y - p0.y, p1.z - p0.z)) < 1.0f) {   // 4265
                 c.x = c.x * kDarken; c.y = c.y * kDarken; c.z = c.z * kDarken;
             }
             const float t = ((vlength(v3(mkx * (sdx - ddx), mky * (sdy - ddy), mkz * (sdz - ddz))) + hit.y) + kTrEps) + sms;
@@ -347,6 +356,16 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
                             break;
                         } else {
                             intersected = true;
+                            if (normal_viz) {   // 4333-4377
+                                V3 n = v3(1.0f, 1.0f, 1.0f);   // finite_diff_normal of a tape scene
+                                if (shading & TSI_SHADE_GRADIENT) {
+                                    const float4 gn = grad[child];
+                                    n = v3(gn.x, gn.y, gn.z);
+                                }
+                                const V3 nv = vnormalize(v3(n.x * 1.0f, n.y * 1.0f, n.z * 1.0f));
+                                c = v3(nv.x * 0.5f + 0.5f, nv.y * 0.5f + 0.5f, nv.z * 0.5f + 0.5f);
+                                break;
+                            }
                             const float tt = (time - cs.y) * 60.0f;
                             c = v3(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
                             break;
@@ -395,10 +414,16 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
         const V3 eye = v3(m[12], m[13], m[14]);
         V3 c;
         float dist;
-        trace_ray(a, stack_lds + threadIdx.x, eye, g, t.time, c, dist, visits);
+        trace_ray(a, stack_lds + threadIdx.x, eye, g, t.time, c, dist, visits, t.shading, t.grad);
         const size_t pi = size_t(y) * t.width + x;
+        // get_color_single: vec4(10, 10, 10, 0) on a miss with normal_viz (4510-4514)
+        float ca = 1.0f;
+        if ((t.shading & TSI_SHADE_NORMALS) && !(dist >= 0.0f)) {
+            c = v3(10.0f, 10.0f, 10.0f);
+            ca = 0.0f;
+        }
         const float r = (0.0f + c.x) / 1.0f, gg = (0.0f + c.y) / 1.0f, b = (0.0f + c.z) / 1.0f;
-        t.rgba[pi] = make_float4(det_pow045(r), det_pow045(gg), det_pow045(b), (0.0f + 1.0f) / 1.0f);
+        t.rgba[pi] = make_float4(det_pow045(r), det_pow045(gg), det_pow045(b), (0.0f + ca) / 1.0f);
         t.depth[pi] = dist;
     }
     uint64_t v = wave_sum64(visits);
@@ -471,21 +496,24 @@ int launch_enqueue_root(const KArgs &a, hipStream_t s) {
 
 // Copy of the nodes in use for an asynchronous trace; the count is read on the device (the
 // host does not wait for the set inversion that produced it).
+// (s_grad: the gradients too, for gradient-normal shading; may be null)
 __global__ void __launch_bounds__(256) k_snapshot(const Counters *cnt, const float4 *st, const float4 *blo,
-                                                  const float4 *bhi, float4 *s_st, float4 *s_blo, float4 *s_bhi,
-                                                  uint64_t nodes) {
+                                                  const float4 *bhi, const float4 *grad, float4 *s_st, float4 *s_blo,
+                                                  float4 *s_bhi, float4 *s_grad, uint64_t nodes) {
     uint64_t used = (uint64_t(cnt->bump) + 1u) * 8u;
     if (used > nodes) used = nodes;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < used; i += uint64_t(gridDim.x) * blockDim.x) {
         s_st[i] = st[i];
         s_blo[i] = blo[i];
         s_bhi[i] = bhi[i];
+        if (s_grad) s_grad[i] = grad[i];
     }
 }
 
-int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, uint64_t nodes, uint32_t grid,
-                    hipStream_t s) {
-    hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, a.cnt, a.st, a.blo, a.bhi, s_st, s_blo, s_bhi, nodes);
+int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, float4 *s_grad, uint64_t nodes,
+                    uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, a.cnt, a.st, a.blo, a.bhi, a.grad, s_st, s_blo, s_bhi,
+                       s_grad, nodes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

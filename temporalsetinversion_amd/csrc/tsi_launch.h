@@ -19,8 +19,8 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
 int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
 int launch_enqueue_root(const KArgs &a, hipStream_t s);
 int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
-int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, uint64_t nodes, uint32_t grid,
-                    hipStream_t s);
+int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, float4 *s_grad, uint64_t nodes,
+                    uint32_t grid, hipStream_t s);
 int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s);
 
 }  // namespace tsi

@@ -70,6 +70,7 @@ struct tsi_ctx {
     hipStream_t tstream = nullptr;
     hipEvent_t ev_snap = nullptr;
     float4 *snap_st = nullptr, *snap_blo = nullptr, *snap_bhi = nullptr;   // voxel_capacity nodes
+    float4 *snap_grad = nullptr;        // gradient normals only
     uint64_t used_nodes = 8;            // nodes in use after the last collected set inversion
     double trace_ms_total = 0.0;
     uint64_t trace_visits_total = 0, trace_count = 0;
@@ -147,6 +148,7 @@ KArgs make_args(const tsi_ctx *c) {
     a.point_sample = c->p.point_sample;
     a.lipschitz = c->p.lipschitz_corrected;
     a.fastenabled = c->p.fastenabled;
+    a.force_deriv = (c->p.shading & TSI_SHADE_NORMALS) && (c->p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
     a.prune = c->prune ? 1 : 0;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) a.prune_rec[i][j] = c->prune_rec[i][j];
@@ -286,7 +288,7 @@ void release(tsi_ctx *c) {
     if (c->tstream) hipStreamSynchronize(c->tstream);
     release_jit(c);
     release_prune(c);
-    dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
+    dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi); dfree(c->snap_grad);
     if (c->ev_snap) { hipEventDestroy(c->ev_snap); c->ev_snap = nullptr; }
     for (auto &f : c->fslot) {
         for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1})
@@ -657,7 +659,10 @@ int tsi_trace_async(tsi_ctx *c) {
         HIP_OK(hipStreamWaitEvent(c->stream, c->tslot[(c->t_issued - 1) % tsi_ctx::kTraceSlots].end, 0));
     KArgs a = make_args(c);
     // snapshot of the nodes in use, in stream order after the last set inversion
-    if (launch_snapshot(a, c->snap_st, c->snap_blo, c->snap_bhi, c->nodes, c->grid_misc, c->stream) != 0)
+    const bool gradient_normals = (c->p.shading & TSI_SHADE_NORMALS) && (c->p.shading & TSI_SHADE_GRADIENT);
+    if (gradient_normals && !c->snap_grad && (rc = dmalloc(&c->snap_grad, c->nodes))) return rc;
+    if (launch_snapshot(a, c->snap_st, c->snap_blo, c->snap_bhi, gradient_normals ? c->snap_grad : nullptr, c->nodes,
+                        c->grid_misc, c->stream) != 0)
         return TSI_E_HIP;
     HIP_OK(hipEventRecord(c->ev_snap, c->stream));
     HIP_OK(hipStreamWaitEvent(c->tstream, c->ev_snap, 0));
@@ -670,6 +675,8 @@ int tsi_trace_async(tsi_ctx *c) {
     t.rgba = c->rgba;
     t.depth = c->depth;
     t.visits = c->visits;
+    t.shading = c->p.shading;
+    t.grad = c->snap_grad;
     a.st = c->snap_st;
     a.blo = c->snap_blo;
     a.bhi = c->snap_bhi;

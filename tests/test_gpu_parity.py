@@ -231,3 +231,22 @@ def test_frames_outside_the_specialised_domain():
         st.update(4.0)
         st.set_inversion()
         assert not st.tape_info()["specialized"]
+
+
+@pytest.mark.parametrize("shading", [1, 3])   # TSI_SHADE_NORMALS, | TSI_SHADE_GRADIENT
+def test_normal_viz_matches_oracle(shading):
+    """normal_viz (compute.wgsl:3503-3504): octree and images bit-identical to the oracle, with
+    the finite-difference normal of a tape scene (1) and with the reverse-pass gradients (3, every
+    cell taking the reverse pass)."""
+    tape = tape_of("spheres")
+    view = T.look_at_inverse((2.2, 1.7, 2.6))
+    for k, g, o, gs, os_, gf, of in run_pair(tape, [0.0, 1 / 60, 0.25], width=64, height=48, trace_every=1,
+                                                localsubdiv=6, view=view, shading=shading):
+        print(f"shading={shading} frame={k} nodes={len(o)} rev={gs['rev_evals']}", flush=True)
+        assert_same_nodes(g, o, f"frame {k}")
+        assert gs["rev_evals"] == os_.rev_evals
+        if shading == 3:
+            assert gs["rev_evals"] == gs["cell_evals"]
+        (grgba, gdepth), (orgba, odepth) = gf, of
+        assert np.array_equal(grgba.view(np.uint32), orgba.view(np.uint32)), f"colour differs in frame {k}"
+        assert np.array_equal(gdepth.view(np.uint32), odepth.view(np.uint32)), f"depth differs in frame {k}"

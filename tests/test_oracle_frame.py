@@ -126,3 +126,84 @@ def test_tree_structure_over_frames():
             for o in range(8):
                 assert (d + 1, 2 * i + (o & 1), 2 * j + ((o >> 1) & 1), 2 * kk + (o >> 2)) in keys
     fr.close()
+
+
+def _frames(tape, shading, times, W=48, H=32, L=5):
+    """(stats, nodes, rgba, depth) per frame of the oracle with the given shading."""
+    import temporalsetinversion_amd as T
+
+    fr = oracle.Frame(tape, localsubdiv=L, voxel_capacity=1 << 20, shading=shading)
+    view = T.look_at_inverse((2.2, 1.7, 2.6))
+    out = []
+    for t in times:
+        t = float(np.float32(t))
+        st = fr.set_inversion(t)
+        rgba, depth, _ = fr.trace(view, W, H, t, T.camera_zdist(H))
+        out.append((st, fr.export(), rgba, depth))
+        fr.enqueue_root()
+    fr.close()
+    return out, view
+
+
+def test_normal_viz_colours():
+    """normal_viz (compute.wgsl:3503-3504, 4265, 4333-4377, 4510-4514): the traversal, hence the
+    depth, is that of the default colouring; hits show normalize(n) * 0.5 + 0.5 after the 0.45
+    gamma, with n = (1, 1, 1) for a tape scene's finite_diff_normal, misses (10, 10, 10, 0)."""
+    tape = oracle.parse_ll(scenes.SCENES["spheres"]())
+    times = [0.0, 1 / 60, 0.25]
+    base, _ = _frames(tape, 0, times)
+    nv, _ = _frames(tape, 1, times)   # ORC_SHADE_NORMALS
+    g = lambda x: oracle.lib().orc_det_pow045(np.float32(x))
+    hit_c = np.float32(g(np.float32(np.float32(1.0) / np.float32(np.sqrt(np.float32(3.0)))) * np.float32(0.5) + np.float32(0.5)))
+    for (_, _, _, d0), (_, _, rgba, d1) in zip(base, nv):
+        assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
+        hit = d1 >= 0
+        assert hit.sum() > 0 and (~hit).sum() > 0
+        assert np.all(rgba[hit][:, 3] == 1.0) and np.all(rgba[~hit][:, 3] == 0.0)
+        assert np.allclose(rgba[hit][:, :3], hit_c, rtol=0, atol=1e-6)
+        assert np.all(rgba[~hit][:, :3] == np.float32(g(10.0)))
+
+
+def test_gradient_normals_follow_the_field():
+    """normal_viz with fast_normals off: every cell takes the reverse pass (3780), the octree is
+    the default one (the derivative only adds dx/dy/dz here), and each hit colour decodes to a unit
+    vector along the float64 gradient of the field at the hit point (the leaf's gradient is taken
+    over the leaf, 10 / 2^7 wide, so within ~35 degrees: cosine above 0.8)."""
+    text = scenes.SCENES["spheres"]()
+    tape = oracle.parse_ll(text)
+    prog = llinterp.Program(text)
+    import temporalsetinversion_amd as T
+
+    times = [0.0, 1 / 60]
+    W, H = 48, 32
+    base, _ = _frames(tape, 0, times, W, H, L=7)
+    grad, view = _frames(tape, 3, times, W, H, L=7)
+    for (s0, n0, _, _), (s1, n1, rgba, depth) in zip(base, grad):
+        assert s1.rev_evals == s1.cell_evals == s0.cell_evals
+        key = lambda n: {(int(a["depth"]), int(a["i"]), int(a["j"]), int(a["k"])): (float(a["f"]), int(a["has_children"]))
+                         for a in n}
+        assert key(n0) == key(n1)
+    # decode the last frame's hit colours: c = (n * 0.5 + 0.5) ^ 0.45
+    t = float(np.float32(times[-1]))
+    zd = T.camera_zdist(H)
+    m = np.asarray(view, np.float32).reshape(16)
+    eye = m[12:15].astype(np.float64)
+    good = total = 0
+    for y in range(H):
+        for x in range(W):
+            if depth[y, x] < 0:
+                continue
+            c = rgba[y, x, :3].astype(np.float64) ** (1.0 / 0.45)
+            n = 2.0 * c - 1.0
+            assert abs(np.linalg.norm(n) - 1.0) < 2e-3, (x, y, n)
+            d = np.array([(x - W / 2.0) + 0.5, -((y - H / 2.0) + 0.5), -zd])
+            d /= np.linalg.norm(d)
+            g = np.array([m[0] * d[0] + m[4] * d[1] + m[8] * d[2], m[1] * d[0] + m[5] * d[1] + m[9] * d[2],
+                          m[2] * d[0] + m[6] * d[1] + m[10] * d[2]])
+            p = eye + float(depth[y, x]) * g
+            h = 1e-4
+            fg = np.array([(prog.eval(*(p + h * e), t)[0] - prog.eval(*(p - h * e), t)[0]) / (2 * h)
+                           for e in np.eye(3)])
+            total += 1
+            good += float(np.dot(n, fg / np.linalg.norm(fg))) > 0.8
+    assert total > 20 and good >= 0.95 * total, (good, total)
