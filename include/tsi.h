@@ -163,18 +163,21 @@ int tsi_trace(tsi_ctx *ctx, float *rgba_out, float *depth_out);
 
 /* The same trace without waiting (no counterpart in the reference, whose render call is
  * synchronous): the nodes in use are copied to a device-side snapshot in stream order after
- * the last set inversion (the count is read on the device: no host wait), and a second stream
- * traces the snapshot while the caller goes on with the next frame's set inversion.  The
- * snapshot holds 48 bytes per node of voxel_capacity, allocated at the first asynchronous trace.
- * tsi_trace_wait waits for every trace in flight and copies the last frame out (rgba / depth may
- * be NULL).  The snapshot of a new tsi_trace_async waits on the device for the previous trace. */
+ * the last set inversion (the count is read on the device: no host wait), and a stream of its
+ * own traces the snapshot while the caller goes on with the next frame's set inversion.  At most
+ * two traces are in flight, each with its own stream, snapshot (48 bytes per node of
+ * voxel_capacity, allocated at the first trace that uses it) and frame, so a trace may start
+ * while the last wavefronts of the previous one still run; a third waits on the host for the
+ * oldest.  tsi_trace_wait waits for every trace in flight and copies the last frame out (rgba /
+ * depth may be NULL). */
 int tsi_trace_async(tsi_ctx *ctx);
 int tsi_trace_wait(tsi_ctx *ctx, float *rgba_out, float *depth_out);
 /* Device time and node visits summed over the traces completed since the context was made. */
 int tsi_trace_totals(tsi_ctx *ctx, double *trace_ms, uint64_t *node_visits, uint64_t *traces);
 
-/* Device pointers of the frame written by the last tsi_trace (valid until the next
- * tsi_trace or tsi_destroy), for callers that keep the frame on the GPU. */
+/* Device pointers of the frame written by the last tsi_trace (valid until the second
+ * tsi_trace after it or tsi_destroy: the two traces in flight write two frames in turn), for
+ * callers that keep the frame on the GPU. */
 int tsi_frame_device(tsi_ctx *ctx, const float **rgba_dev, const float **depth_dev);
 
 /* buf_main (src/main.rs:683-690, compute.wgsl:4690-4707): re-enqueue the root so the

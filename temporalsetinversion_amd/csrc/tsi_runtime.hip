@@ -54,10 +54,6 @@ struct tsi_ctx {
     hipModule_t mod[2] = {nullptr, nullptr};
     hipFunction_t eval_fn[2] = {nullptr, nullptr};   // forward-only list, reverse list
     tsi_tape_info info{};
-    float4 *rgba = nullptr;
-    float *depth = nullptr;
-    size_t frame_px = 0;
-    unsigned long long *visits = nullptr;
     std::vector<tsi_gpu_instruction> tape;
     Lowered low{};
     bool have_tape = false;
@@ -65,12 +61,6 @@ struct tsi_ctx {
     uint32_t grid_fwd = 0, grid_rev = 0, grid_misc = 0;
     uint64_t last_visits = 0;
     double last_trace_ms = 0.0;
-    // asynchronous trace: a second stream traces a device-side snapshot of the octree while the
-    // next frame's set inversion runs on the main stream
-    hipStream_t tstream = nullptr;
-    hipEvent_t ev_snap = nullptr;
-    float4 *snap_st = nullptr, *snap_blo = nullptr, *snap_bhi = nullptr;   // voxel_capacity nodes
-    float4 *snap_grad = nullptr;        // gradient normals only
     uint64_t used_nodes = 8;            // nodes in use after the last collected set inversion
     double trace_ms_total = 0.0;
     uint64_t trace_visits_total = 0, trace_count = 0;
@@ -80,14 +70,25 @@ struct tsi_ctx {
         hipEvent_t begin = nullptr, end = nullptr, eval0 = nullptr, eval1 = nullptr;
         Counters *h = nullptr;
     };
+    // a trace runs on its slot's stream over its slot's device-side snapshot of the octree, so it
+    // overlaps the next frame's set inversion on the main stream and the previous trace's last
+    // wavefronts on the other slot's stream
     struct TraceSlot {
-        hipEvent_t begin = nullptr, end = nullptr;
+        hipEvent_t begin = nullptr, end = nullptr, snap = nullptr;
         unsigned long long *visits_h = nullptr;
+        hipStream_t stream = nullptr;
+        float4 *st = nullptr, *blo = nullptr, *bhi = nullptr;   // snapshot, voxel_capacity nodes
+        float4 *grad = nullptr;                                 // gradient normals only
+        float4 *rgba = nullptr;                                 // the frame it writes
+        float *depth = nullptr;
+        size_t px = 0;
+        unsigned long long *visits = nullptr;
     };
     static constexpr uint32_t kFrameSlots = 2, kTraceSlots = 2;
     FrameSlot fslot[kFrameSlots];
     TraceSlot tslot[kTraceSlots];
     uint64_t f_issued = 0, f_done = 0, t_issued = 0, t_done = 0;
+    uint32_t last_tslot = 0;   // the slot of the last trace issued: the frame read back
     // the domain the loaded tape kernels are specialised for (they leave out fix where it is the
     // identity there), the root box of the last reset and the t-windows seen since
     bool jit_specialized = false;
@@ -285,27 +286,27 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
 
 void release(tsi_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->tstream) hipStreamSynchronize(c->tstream);
+    for (auto &t : c->tslot)
+        if (t.stream) hipStreamSynchronize(t.stream);
     release_jit(c);
     release_prune(c);
-    dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi); dfree(c->snap_grad);
-    if (c->ev_snap) { hipEventDestroy(c->ev_snap); c->ev_snap = nullptr; }
     for (auto &f : c->fslot) {
         for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1})
             if (*e) { hipEventDestroy(*e); *e = nullptr; }
         if (f.h) { hipHostFree(f.h); f.h = nullptr; }
     }
     for (auto &t : c->tslot) {
-        for (hipEvent_t *e : {&t.begin, &t.end})
+        for (hipEvent_t *e : {&t.begin, &t.end, &t.snap})
             if (*e) { hipEventDestroy(*e); *e = nullptr; }
         if (t.visits_h) { hipHostFree(t.visits_h); t.visits_h = nullptr; }
+        dfree(t.st); dfree(t.blo); dfree(t.bhi); dfree(t.grad); dfree(t.rgba); dfree(t.depth); dfree(t.visits);
+        if (t.stream) { hipStreamDestroy(t.stream); t.stream = nullptr; }
     }
-    if (c->tstream) { hipStreamDestroy(c->tstream); c->tstream = nullptr; }
     dfree(c->st); dfree(c->blo); dfree(c->bhi); dfree(c->grad);
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
     dfree(c->pending); dfree(c->free_stack); dfree(c->cnt); dfree(c->stack);
-    dfree(c->fprog); dfree(c->rprog); dfree(c->rgba); dfree(c->depth); dfree(c->visits);
+    dfree(c->fprog); dfree(c->rprog);
     for (auto &e : c->ev) if (e) { hipEventDestroy(e); e = nullptr; }
     if (c->stream) { hipStreamDestroy(c->stream); c->stream = nullptr; }
 }
@@ -367,9 +368,11 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming) != hipSuccess) {
+        bool sok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+        for (auto &t : c->tslot)
+            sok = sok && hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&t.snap, hipEventDisableTiming) == hipSuccess;
+        if (!sok) {
             rc = TSI_E_HIP;
             break;
         }
@@ -403,7 +406,8 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         if (rc) break;
         if ((rc = dmalloc(&c->dlists[0], c->dlist_cap)) || (rc = dmalloc(&c->dlists[1], c->dlist_cap)) ||
             (rc = dmalloc(&c->pending, c->nblocks)) || (rc = dmalloc(&c->free_stack, c->nblocks)) ||
-            (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->visits, 1)))
+            (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->tslot[0].visits, 1)) ||
+            (rc = dmalloc(&c->tslot[1].visits, 1)))
             break;
         if (hipMemsetAsync(c->cnt, 0, sizeof(Counters), c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
         c->grid_fwd = uint32_t(c->num_cu) * 8u;
@@ -530,7 +534,7 @@ int keep_domain(tsi_ctx *c) {
     for (int k = 0; k < 3; ++k) inside = inside && c->root_box[k] >= d.box[k] && c->root_box[3 + k] <= d.box[3 + k];
     if (inside) return TSI_OK;
     HIP_OK(hipStreamSynchronize(c->stream));   // frames in flight use the loaded kernels
-    HIP_OK(hipStreamSynchronize(c->tstream));
+    for (auto &t : c->tslot) HIP_OK(hipStreamSynchronize(t.stream));
     std::vector<FwdIns> fwd;
     std::vector<RevIns> rev;
     Lowered L;
@@ -637,55 +641,53 @@ int tsi_trace_async(tsi_ctx *c) {
     const uint32_t W = c->u.resolution[0], H = c->u.resolution[1];
     if (W == 0 || H == 0) return TSI_E_ARG;
     int rc;
+    // the slot's previous trace is collected first: its snapshot and frame are then free
     if (c->t_issued - c->t_done >= tsi_ctx::kTraceSlots && (rc = harvest_trace(c))) return rc;
+    const uint32_t k = uint32_t(c->t_issued % tsi_ctx::kTraceSlots);
+    tsi_ctx::TraceSlot &ts = c->tslot[k];
     const size_t px = size_t(W) * H;
-    if (px > c->frame_px) {   // the traces in flight write the old buffers
-        if ((rc = harvest_all(c))) return rc;
-        dfree(c->rgba);
-        dfree(c->depth);
-        c->frame_px = 0;
-        if ((rc = dmalloc(&c->rgba, px)) || (rc = dmalloc(&c->depth, px))) return rc;
-        c->frame_px = px;
+    if (px > ts.px) {
+        dfree(ts.rgba);
+        dfree(ts.depth);
+        ts.px = 0;
+        if ((rc = dmalloc(&ts.rgba, px)) || (rc = dmalloc(&ts.depth, px))) return rc;
+        ts.px = px;
     }
-    if (!c->snap_st) {
-        if ((rc = dmalloc(&c->snap_st, c->nodes)) || (rc = dmalloc(&c->snap_blo, c->nodes)) ||
-            (rc = dmalloc(&c->snap_bhi, c->nodes))) {
-            dfree(c->snap_st); dfree(c->snap_blo); dfree(c->snap_bhi);
+    if (!ts.st) {
+        if ((rc = dmalloc(&ts.st, c->nodes)) || (rc = dmalloc(&ts.blo, c->nodes)) || (rc = dmalloc(&ts.bhi, c->nodes))) {
+            dfree(ts.st); dfree(ts.blo); dfree(ts.bhi);
             return rc;
         }
     }
-    // the snapshot is rewritten only after the previous trace has read it (a device-side wait)
-    if (c->t_issued != 0)
-        HIP_OK(hipStreamWaitEvent(c->stream, c->tslot[(c->t_issued - 1) % tsi_ctx::kTraceSlots].end, 0));
     KArgs a = make_args(c);
     // snapshot of the nodes in use, in stream order after the last set inversion
     const bool gradient_normals = (c->p.shading & TSI_SHADE_NORMALS) && (c->p.shading & TSI_SHADE_GRADIENT);
-    if (gradient_normals && !c->snap_grad && (rc = dmalloc(&c->snap_grad, c->nodes))) return rc;
-    if (launch_snapshot(a, c->snap_st, c->snap_blo, c->snap_bhi, gradient_normals ? c->snap_grad : nullptr, c->nodes,
-                        c->grid_misc, c->stream) != 0)
+    if (gradient_normals && !ts.grad && (rc = dmalloc(&ts.grad, c->nodes))) return rc;
+    if (launch_snapshot(a, ts.st, ts.blo, ts.bhi, gradient_normals ? ts.grad : nullptr, c->nodes, c->grid_misc,
+                        c->stream) != 0)
         return TSI_E_HIP;
-    HIP_OK(hipEventRecord(c->ev_snap, c->stream));
-    HIP_OK(hipStreamWaitEvent(c->tstream, c->ev_snap, 0));
+    HIP_OK(hipEventRecord(ts.snap, c->stream));
+    HIP_OK(hipStreamWaitEvent(ts.stream, ts.snap, 0));
     TraceArgs t{};
     std::memcpy(t.view, c->u.view, sizeof t.view);
     t.width = W;
     t.height = H;
     t.time = c->u.time;
     t.zdist = tsi_camera_zdist(H);
-    t.rgba = c->rgba;
-    t.depth = c->depth;
-    t.visits = c->visits;
+    t.rgba = ts.rgba;
+    t.depth = ts.depth;
+    t.visits = ts.visits;
     t.shading = c->p.shading;
-    t.grad = c->snap_grad;
-    a.st = c->snap_st;
-    a.blo = c->snap_blo;
-    a.bhi = c->snap_bhi;
-    tsi_ctx::TraceSlot &ts = c->tslot[c->t_issued % tsi_ctx::kTraceSlots];
-    HIP_OK(hipMemsetAsync(c->visits, 0, sizeof(unsigned long long), c->tstream));
-    HIP_OK(hipEventRecord(ts.begin, c->tstream));
-    if (launch_trace(a, t, c->tstream) != 0) return TSI_E_HIP;
-    HIP_OK(hipMemcpyAsync(ts.visits_h, c->visits, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->tstream));
-    HIP_OK(hipEventRecord(ts.end, c->tstream));
+    t.grad = ts.grad;
+    a.st = ts.st;
+    a.blo = ts.blo;
+    a.bhi = ts.bhi;
+    HIP_OK(hipMemsetAsync(ts.visits, 0, sizeof(unsigned long long), ts.stream));
+    HIP_OK(hipEventRecord(ts.begin, ts.stream));
+    if (launch_trace(a, t, ts.stream) != 0) return TSI_E_HIP;
+    HIP_OK(hipMemcpyAsync(ts.visits_h, ts.visits, sizeof(unsigned long long), hipMemcpyDeviceToHost, ts.stream));
+    HIP_OK(hipEventRecord(ts.end, ts.stream));
+    c->last_tslot = k;
     c->t_issued += 1;
     return TSI_OK;
 }
@@ -694,10 +696,11 @@ int tsi_trace_wait(tsi_ctx *c, float *rgba_out, float *depth_out) {
     if (!c) return TSI_E_ARG;
     int rc;
     if ((rc = harvest_all(c))) return rc;
+    const tsi_ctx::TraceSlot &ts = c->tslot[c->last_tslot];
     const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
-    if ((rgba_out || depth_out) && px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
-    if (rgba_out) HIP_OK(hipMemcpy(rgba_out, c->rgba, px * sizeof(float4), hipMemcpyDeviceToHost));
-    if (depth_out) HIP_OK(hipMemcpy(depth_out, c->depth, px * sizeof(float), hipMemcpyDeviceToHost));
+    if ((rgba_out || depth_out) && (c->t_issued == 0 || px > ts.px)) return TSI_E_ARG;   // nothing traced at this size
+    if (rgba_out) HIP_OK(hipMemcpy(rgba_out, ts.rgba, px * sizeof(float4), hipMemcpyDeviceToHost));
+    if (depth_out) HIP_OK(hipMemcpy(depth_out, ts.depth, px * sizeof(float), hipMemcpyDeviceToHost));
     return TSI_OK;
 }
 
@@ -719,11 +722,12 @@ int tsi_frame_rgba8(tsi_ctx *c, uint8_t *rgba8_out) {
     if (!c || !rgba8_out) return TSI_E_ARG;
     int rc;
     if ((rc = harvest_all(c))) return rc;
+    const tsi_ctx::TraceSlot &ts = c->tslot[c->last_tslot];
     const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
-    if (px == 0 || px > c->frame_px) return TSI_E_ARG;   // nothing traced at this size
+    if (px == 0 || c->t_issued == 0 || px > ts.px) return TSI_E_ARG;   // nothing traced at this size
     uchar4 *tmp = nullptr;
     if ((rc = dmalloc(&tmp, px))) return rc;
-    rc = launch_rgba8(c->rgba, tmp, uint32_t(px), c->stream) == 0 ? TSI_OK : TSI_E_HIP;
+    rc = launch_rgba8(ts.rgba, tmp, uint32_t(px), c->stream) == 0 ? TSI_OK : TSI_E_HIP;
     if (rc == TSI_OK && hipMemcpyAsync(rgba8_out, tmp, px * 4u, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         rc = TSI_E_HIP;
     if (rc == TSI_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TSI_E_HIP;
@@ -747,8 +751,9 @@ int tsi_save_frame_png(tsi_ctx *c, const char *path) {
 
 int tsi_frame_device(tsi_ctx *c, const float **rgba_dev, const float **depth_dev) {
     if (!c) return TSI_E_ARG;
-    if (rgba_dev) *rgba_dev = reinterpret_cast<const float *>(c->rgba);
-    if (depth_dev) *depth_dev = c->depth;
+    const tsi_ctx::TraceSlot &ts = c->tslot[c->last_tslot];
+    if (rgba_dev) *rgba_dev = reinterpret_cast<const float *>(ts.rgba);
+    if (depth_dev) *depth_dev = ts.depth;
     return TSI_OK;
 }
 
