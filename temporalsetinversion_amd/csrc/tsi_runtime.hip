@@ -410,8 +410,11 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             (rc = dmalloc(&c->tslot[1].visits, 1)))
             break;
         if (hipMemsetAsync(c->cnt, 0, sizeof(Counters), c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
-        c->grid_fwd = uint32_t(c->num_cu) * 8u;
-        c->grid_rev = uint32_t(c->num_cu) * 2u;
+        // level kernels stride over their frontier; 4 blocks (16 wavefronts) per CU for both lists
+        // (measured: the reverse list of a large frontier needs 4 wavefronts per SIMD to hide its
+        // latency, more only grows the saved-operand stack; the forward-only list is insensitive)
+        c->grid_fwd = uint32_t(c->num_cu) * 4u;
+        c->grid_rev = uint32_t(c->num_cu) * 4u;
         c->grid_misc = uint32_t(c->num_cu) * 4u;
         if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
     } while (0);
