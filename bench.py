@@ -101,8 +101,9 @@ def summarize_profile(args) -> None:
     lines.append(f"#   python3 bench.py --workload {W} --summarize <d>/kt/kt_results.db --pmc-fetch <d>/FETCH_SIZE/pmc_results.db "
                  "--pmc-write <d>/WRITE_SIZE/pmc_results.db --copy-fetch <d>/copy_FETCH_SIZE/pmc_results.db "
                  "--copy-write <d>/copy_WRITE_SIZE/pmc_results.db")
-    lines += ["#", f"# {frames} frames (warm-up + timed); one forward-only and one reverse level kernel per octree",
-              "# level (tsi_tape_eval_fwd / _rev), one snapshot + trace per frame on the second stream.", ""]
+    lines += ["#", f"# {frames} frames (warm-up + timed); one level kernel per octree level running the forward-only",
+              "# and the reverse list side by side (tsi_tape_eval_level), one snapshot + trace per frame on",
+              "# a trace stream.", ""]
     lines.append(f"{'kernel':<44} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
     for k, (n, t) in sorted(kt.items(), key=lambda x: -x[1][1]):
         lines.append(f"{k[:44]:<44} {n:>10} {t / 1e6:>10.3f} {t / n / 1e3:>9.2f} {100 * t / total:>6.1f}%")
@@ -119,7 +120,7 @@ def summarize_profile(args) -> None:
         lines += ["", f"copy calibration ({args.copy_mib} MiB device-to-device): FETCH_SIZE scale {sf:.3f}, "
                       f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024)",
                   f"{'kernel':<24} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'hbm_bytes/disp':>15}"]
-        for k in ("tsi_tape_eval_fwd", "tsi_tape_eval_rev", "tsi::k_trace", "tsi::k_snapshot"):
+        for k in ("tsi_tape_eval_level", "tsi::k_trace", "tsi::k_snapshot"):
             if k in f and k in w:
                 nf, vf = f[k]
                 nw, vw = w[k]
@@ -327,8 +328,9 @@ def main():
         "rev_evals_per_frame": tot["rev"] / args.steps,
         "frames_per_s": args.steps / wall,
         "roofline": {"bound": "valu",
-                     "kernel": ("tsi_tape_eval_fwd/_rev" if info["engine"] == "compiled" else "k_eval<false>/<true>")
-                               + " (one forward-only and one reverse level kernel per octree level)",
+                     "kernel": ("tsi_tape_eval_level (one launch per octree level: forward-only and reverse lists)"
+                                if info["engine"] == "compiled" else
+                                "k_eval<false>/<true> (one forward-only and one reverse launch per octree level)"),
                      "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "T f32-ops/s",
                      "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic, "traffic_unit": "HBM bytes per launch",
                      "traffic_source": traffic_src,

@@ -109,7 +109,7 @@ typedef struct {
     uint64_t deallocs;            /* node blocks freed by the dealloc cascade */
     uint64_t enqueued;            /* cells enqueued for subdivision */
     uint64_t allocated_nodes;     /* live pool nodes after the frame */
-    uint64_t eval_launches;
+    uint64_t eval_launches;       /* tape-evaluation kernel launches */
     uint64_t pruned_instr;        /* sum over evaluated cells of instructions skipped by pruning */
     uint64_t prune_launches;      /* group-program generator launches */
 } tsi_stats;

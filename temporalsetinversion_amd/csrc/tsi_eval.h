@@ -53,8 +53,10 @@ TSI_DI uint64_t dealloc_step(const KArgs &a, int step, uint32_t wave, uint32_t n
     return freed;
 }
 
+// One level of one frontier list: wavefront `wave` of `nwaves` strides over the list; `bwave`
+// indexes the wavefront's own buffers (saved-operand stack, choice rows), unique in the launch.
 template <bool REV, class Engine>
-TSI_DI void eval_level(const KArgs &a, int level, float time) {
+TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, uint32_t nwaves, uint32_t bwave_) {
     const int which = REV ? 1 : 0;
     Counters *cnt = a.cnt;
     // a list that overflowed (flagged by wave_append) holds list_cap valid entries
@@ -65,12 +67,12 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
     const uint32_t lane = lane_id();
     const uint32_t ps = lane >> 3, o = lane & 7u;
     const uint32_t lx = o & 1u, ly = (o >> 1) & 1u, lz = o >> 2;
-    // wave index made visibly uniform: the grid-stride loop and everything inside it then
+    // wave indices made visibly uniform: the grid-stride loop and everything inside it then
     // compile as scalar control flow
-    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_);
+    const uint32_t bwave = __builtin_amdgcn_readfirstlane(bwave_);
     // this wavefront's saved-operand stack (uniform base; lane l uses column l of each row)
-    float2 *stk = REV ? a.stack + size_t(wave) * a.stack_len * 64 : nullptr;
+    float2 *stk = REV ? a.stack + size_t(bwave) * a.stack_len * 64 : nullptr;
     const float eps = a.epsilon;
     const bool prune = Engine::kPrune && a.prune != 0;
     // this level's parents' liveness records (written by the previous level) and the next
@@ -78,7 +80,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
     const uint32_t *prec_in = a.prune_rec[level & 1][which];
     uint32_t *prec_f = a.prune_rec[(level + 1) & 1][0];
     uint32_t *prec_r = a.prune_rec[(level + 1) & 1][1];
-    Engine E(a, wave, lane);
+    Engine E(a, bwave, lane);
     // register file of engines that keep values in a uniformly indexed array (a kernel-local
     // array, so that it is promoted to VGPRs; unused by straight-line engines)
     Slots Rlo, Rhi;
@@ -313,6 +315,31 @@ TSI_DI void eval_level(const KArgs &a, int level, float time) {
         if (n_pruned) atomicAdd((unsigned long long *)&cnt->stats[kStatPruned], (unsigned long long)n_pruned);
         if (n_freed) atomicAdd((unsigned long long *)&cnt->stats[kStatDeallocs], (unsigned long long)n_freed);
     }
+}
+
+// A level kernel that is one launch for one list: wavefront index and count from the grid.
+template <bool REV, class Engine>
+TSI_DI void eval_level(const KArgs &a, int level, float time) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    eval_level<REV, Engine>(a, level, time, wave, (gridDim.x * blockDim.x) >> 6, wave);
+}
+
+// Both lists of a level in one launch of `blocks` blocks of 4 wavefronts: the number of blocks
+// that take the forward-only list (and the level's dealloc step), the rest taking the reverse
+// list.  Each list gets the blocks its work fills when both fit, else a share of the grid in
+// proportion to its work; every block computes the same split from the counts the previous
+// launches wrote.
+TSI_DI uint32_t level_split(const KArgs &a, int level, uint32_t blocks) {
+    const Counters *cnt = a.cnt;
+    uint32_t cf = min(cnt->list_count[level][0], a.list_cap);
+    if (level > 0) cf = max(cf, min(cnt->dlist_count[level], a.dlist_cap));
+    const uint32_t cr = min(cnt->list_count[level][1], a.list_cap);
+    const uint32_t bf = (cf + 31u) / 32u, br = (cr + 31u) / 32u;   // 8 entries per wavefront
+    if (bf + br <= blocks) return bf;
+    if (bf == 0u) return 0u;
+    if (br == 0u) return blocks;
+    const uint32_t nf = uint32_t(uint64_t(blocks) * bf / (uint64_t(bf) + br));
+    return nf < 1u ? 1u : (nf > blocks - 1u ? blocks - 1u : nf);
 }
 
 }  // namespace tsi

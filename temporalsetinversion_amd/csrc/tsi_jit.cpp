@@ -762,6 +762,44 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     return o;
 }
 
+namespace {
+
+// The definitions of a generated module: what it holds inside namespace tsi, without the
+// includes and the kernel.
+std::string module_body(const std::string &src) {
+    const std::string open = "namespace tsi {\n", close = "}  // namespace tsi\n";
+    const size_t a = src.find(open);
+    const size_t b = src.rfind(close);
+    if (a == std::string::npos || b == std::string::npos || b < a) throw std::runtime_error("module layout");
+    return src.substr(a + open.size(), b - a - open.size());
+}
+
+}  // namespace
+
+std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
+                             const std::vector<RevIns> &rev, const Lowered &L, bool prune, const JitDomain *domain) {
+    const std::string f = jit_source(tape, n, fwd, rev, L, false, prune, domain);
+    const std::string r = jit_source(tape, n, fwd, rev, L, true, prune, domain);
+    std::string o = "// generated by tsi_jit.cpp: straight-line evaluation of one tape, both lists of a level\n";
+    o += "#include \"tsi_eval.h\"\n#include \"tsi_rules.h\"\n\nnamespace tsi {\nnamespace jf {   // forward-only list\n";
+    o += module_body(f);
+    o += "}  // namespace jf\nnamespace jr {   // reverse list\n";
+    o += module_body(r);
+    o += "}  // namespace jr\n}  // namespace tsi\n\n";
+    // one launch per level: the forward-only list and the reverse list run side by side, each
+    // on its share of the grid (level_split), instead of one after the other
+    o += "extern \"C\" __global__ void __launch_bounds__(256) tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n"
+         "    const uint32_t nb = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6;\n"
+         "    const uint32_t nf = __builtin_amdgcn_readfirstlane(tsi::level_split(a, level, nb));\n"
+         "    const uint32_t bwave = b * 4u + w;\n"
+         "    if (b < nf)\n"
+         "        tsi::eval_level<false, tsi::jf::TapeEngine>(a, level, time, bwave, nf * 4u, bwave);\n"
+         "    else\n"
+         "        tsi::eval_level<true, tsi::jr::TapeEngine>(a, level, time, (b - nf) * 4u + w, (nb - nf) * 4u, bwave);\n"
+         "}\n";
+    return o;
+}
+
 }  // namespace tsi
 
 // ---------------------------------------------------------------- compilation

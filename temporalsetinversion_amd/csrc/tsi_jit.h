@@ -23,13 +23,19 @@ struct JitDomain { float box[6]; float t0, t1; };
 std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
                            const JitDomain &d);
 
-// HIP source of the level kernel for one tape: tsi_tape_eval_fwd (forward-only frontier list)
-// or tsi_tape_eval_rev (forward with saved operands, then reverse).  prune: regions dead for
+// HIP source of the code of one list for one tape: the forward-only frontier list, or
+// (rev_kernel) forward with saved operands, then reverse; with a kernel of its own
+// (tsi_tape_eval_fwd / tsi_tape_eval_rev), which jit_source_level replaces by one for both.  prune: regions dead for
 // all of a wavefront's parents are skipped and decided MIN/MAX become copies (liveness records
 // written for enqueued cells).
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
                        const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
                        const JitDomain *domain = nullptr);
+// Both level kernels' code in one module with one kernel, tsi_tape_eval_level(KArgs, level,
+// time), that runs a level's forward-only and reverse lists side by side (blockDim 256).
+std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
+                             const std::vector<RevIns> &rev, const Lowered &L, bool prune,
+                             const JitDomain *domain = nullptr);
 
 // Compiles `src` for gfx950 with hiprtc (the library's kernel headers are embedded), or finds
 // the code object in this process's cache or the on-disk cache ($TSI_JIT_CACHE, default

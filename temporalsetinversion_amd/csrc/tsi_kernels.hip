@@ -453,7 +453,7 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a);
     if (e_eval0) hipEventRecord(e_eval0, s);
     for (int d = 0; d < fl.levels; ++d) {
-        const bool compiled = fl.tape_eval[0] && fl.tape_eval[1];
+        const bool compiled = fl.tape_eval != nullptr;
         if (a.prune && d > 0 && !compiled) {   // the interpreter's group programs
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 0);
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 1);
@@ -463,10 +463,9 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
             int level = d;
             float t = fl.time;
             void *args[] = {&ka, &level, &t};
-            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval[0], fl.grid_fwd, 1, 1, 256, 1, 1, 0, s, args,
-                                      nullptr) != hipSuccess ||
-                hipModuleLaunchKernel((hipFunction_t)fl.tape_eval[1], fl.grid_rev, 1, 1, 256, 1, 1, 0, s, args,
-                                      nullptr) != hipSuccess)
+            // both lists of the level in one launch (tsi_tape_eval_level splits the grid)
+            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, fl.grid_fwd + fl.grid_rev, 1, 1, 256, 1, 1, 0, s,
+                                      args, nullptr) != hipSuccess)
                 return -1;
         } else {
             hipLaunchKernelGGL(k_eval<false>, dim3(fl.grid_fwd), dim3(256), 0, s, a, d, fl.time);

@@ -12,7 +12,7 @@ struct FrameLaunch {
     int levels;
     float time;
     uint32_t grid_fwd, grid_rev, grid_misc;
-    void *tape_eval[2];   // compiled level kernels (hipFunction_t) of the tape, or null: k_eval
+    void *tape_eval;   // the tape's compiled level kernel (hipFunction_t), or null: k_eval
 };
 
 int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1);

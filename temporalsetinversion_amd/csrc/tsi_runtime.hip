@@ -10,7 +10,6 @@
 #include <new>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "tsi.h"
@@ -196,7 +195,7 @@ int alloc_prune(tsi_ctx *c, const Lowered &L, const std::vector<FwdIns> &fwd) {
         words = uint64_t(Y.record_words) * cap;
         choice_len = Y.mode_words ? Y.mode_words : 1;
     }
-    const uint32_t waves = (c->grid_fwd > c->grid_rev ? c->grid_fwd : c->grid_rev) * 4u;
+    const uint32_t waves = (c->grid_fwd + c->grid_rev) * 4u;   // the compiled level kernel's grid
     // worst case: every group of every list keeps the whole program, twice (fwd + rev)
     const uint64_t per_group = 2ull * ((L.nfwd + 3) / 4 * 4 + 8);
     uint64_t pool = 2ull * (cap / 8) * per_group;
@@ -241,38 +240,31 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
              const std::vector<RevIns> &rev, const Lowered &L, const JitDomain *domain) {
     release_jit(c);
     c->jit_specialized = false;
-    std::string src[2], log[2];
-    std::vector<char> code[2];
-    bool hit[2] = {false, false};
-    double ms[2] = {0.0, 0.0};
-    int rc[2] = {TSI_OK, TSI_OK};
+    std::string src, log;
+    std::vector<char> code;
+    bool hit = false;
+    double ms = 0.0;
+    int rc = TSI_OK;
     try {
-        for (int k = 0; k < 2; ++k) src[k] = jit_source(tape, n, fwd, rev, L, k == 1, c->prune, domain);
-        // the two kernels compile in parallel
-        std::thread t([&] { rc[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
-        rc[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
-        t.join();
+        src = jit_source_level(tape, n, fwd, rev, L, c->prune, domain);
+        rc = jit_compile(src, code, log, hit, ms);
     } catch (const std::runtime_error &) {   // the generator cannot lay out this tape
         return TSI_E_JIT;
     } catch (...) {
         return TSI_E_OOM;
     }
-    c->info.compile_ms = ms[0] > ms[1] ? ms[0] : ms[1];
-    c->info.cache_hit = hit[0] && hit[1];
-    for (int k = 0; k < 2; ++k) {
-        if (rc[k] != TSI_OK) {
-            std::strncpy(c->info.log, log[k].c_str(), sizeof c->info.log - 1);
-            c->info.log[sizeof c->info.log - 1] = 0;
-            return rc[k];
-        }
+    c->info.compile_ms = ms;
+    c->info.cache_hit = hit;
+    if (rc != TSI_OK) {
+        std::strncpy(c->info.log, log.c_str(), sizeof c->info.log - 1);
+        c->info.log[sizeof c->info.log - 1] = 0;
+        return rc;
     }
-    static const char *const names[2] = {"tsi_tape_eval_fwd", "tsi_tape_eval_rev"};
-    for (int k = 0; k < 2; ++k) {
-        if (hipModuleLoadData(&c->mod[k], code[k].data()) != hipSuccess ||
-            hipModuleGetFunction(&c->eval_fn[k], c->mod[k], names[k]) != hipSuccess) {
-            release_jit(c);
-            return TSI_E_HIP;
-        }
+    // one kernel runs both lists of a level (eval_fn[1] stays null)
+    if (hipModuleLoadData(&c->mod[0], code.data()) != hipSuccess ||
+        hipModuleGetFunction(&c->eval_fn[0], c->mod[0], "tsi_tape_eval_level") != hipSuccess) {
+        release_jit(c);
+        return TSI_E_HIP;
     }
     c->jit_specialized = domain != nullptr;
     c->info.specialized = domain ? 1u : 0u;
@@ -461,7 +453,8 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
     HIP_OK(hipMemcpy(c->rprog, rev.data(), rev.size() * sizeof(RevIns), hipMemcpyHostToDevice));
     // per wavefront: L.stack rows of 64 lanes, plus one spare row (the reverse pass loads two
     // intervals per record whatever the record saved)
-    size_t need = size_t(c->grid_rev) * 4u * 64u * size_t(L.stack + 1) * sizeof(float2);
+    // (one per wavefront of the compiled level kernel's grid, which any list may fill)
+    size_t need = size_t(c->grid_fwd + c->grid_rev) * 4u * 64u * size_t(L.stack + 1) * sizeof(float2);
     if (need > c->stack_bytes) {
         dfree(c->stack);
         c->stack_bytes = 0;
@@ -561,7 +554,7 @@ int tsi_set_inversion_async(tsi_ctx *c) {
     tsi_ctx::FrameSlot &f = c->fslot[c->f_issued % tsi_ctx::kFrameSlots];
     KArgs a = make_args(c);
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
-                   {(void *)c->eval_fn[0], (void *)c->eval_fn[1]}};
+                   (void *)c->eval_fn[0]};
     HIP_OK(hipEventRecord(f.begin, c->stream));
     if (launch_frame(a, fl, c->stream, f.eval0, f.eval1) != 0) return TSI_E_HIP;
     HIP_OK(hipMemcpyAsync(f.h, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
@@ -594,7 +587,8 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         stats->deallocs = h.stats[kStatDeallocs];
         stats->enqueued = h.stats[kStatEnqueued];
         stats->allocated_nodes = h.stats[kStatLive] * 8u;
-        stats->eval_launches = uint64_t(2 * c->p.localsubdiv);
+        // the compiled engine runs both lists of a level in one launch, the interpreter in two
+        stats->eval_launches = uint64_t((c->eval_fn[0] ? 1 : 2) * c->p.localsubdiv);
         stats->prune_launches = c->prune && !c->eval_fn[0] ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
     }
     c->used_nodes = (uint64_t(h.bump) + 1u) * 8u;
@@ -787,28 +781,19 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t fl
             info.slots = L.slots;
             info.prune_class = uint32_t(L.prune_class);
             info.engine = 1;
-            std::string src[2], log[2];
-            std::vector<char> code[2];
-            bool hit[2] = {false, false};
-            double ms[2] = {0.0, 0.0};
-            int r[2] = {TSI_OK, TSI_OK};
-            for (int k = 0; k < 2; ++k)
-                src[k] = jit_source(tape, n, fwd, rev, L, k == 1, prune_wanted(flags, L), &kDefaultDomain);
+            std::string src, log;
+            std::vector<char> code;
+            bool hit = false;
+            double ms = 0.0;
+            src = jit_source_level(tape, n, fwd, rev, L, prune_wanted(flags, L), &kDefaultDomain);
             info.specialized = 1;
             info.fix_kept = n - 4u;
             const std::vector<char> ff = fix_free(tape, n, fwd, kDefaultDomain);
             for (uint32_t j = 4; j < n; ++j) info.fix_kept -= ff[j] ? 1u : 0u;
-            std::thread t([&] { r[1] = jit_compile(src[1], code[1], log[1], hit[1], ms[1]); });
-            r[0] = jit_compile(src[0], code[0], log[0], hit[0], ms[0]);
-            t.join();
-            info.compile_ms = ms[0] > ms[1] ? ms[0] : ms[1];
-            info.cache_hit = hit[0] && hit[1];
-            for (int k = 0; k < 2 && rc == TSI_OK; ++k) {
-                if (r[k] != TSI_OK) {
-                    rc = r[k];
-                    std::strncpy(info.log, log[k].c_str(), sizeof info.log - 1);
-                }
-            }
+            rc = jit_compile(src, code, log, hit, ms);
+            info.compile_ms = ms;
+            info.cache_hit = hit;
+            if (rc != TSI_OK) std::strncpy(info.log, log.c_str(), sizeof info.log - 1);
         }
     } catch (const std::runtime_error &) {
         return TSI_E_JIT;
