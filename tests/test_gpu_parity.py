@@ -212,6 +212,38 @@ def test_frames_in_flight_match_blocking_frames():
     assert np.array_equal(img.view(np.uint32), want_img[-1].view(np.uint32))
 
 
+def test_overlapping_traces_keep_each_frame():
+    """Two traces in flight use the two trace slots, each with its own snapshot and frame: trace k
+    is issued while trace k-1 may still run and while frame k+1's set inversion rewrites the
+    octree, and the image and depth read back after every second frame equal that frame's
+    blocking trace bit for bit."""
+    tape = tape_of("spheres")
+    view = T.look_at_inverse((2.2, 1.7, 2.6))
+    times = [float(np.float32(k / 60.0)) for k in range(8)]
+    kw = dict(width=64, height=48, localsubdiv=6, voxel_capacity=1 << 20)
+    want = []
+    with T.State(tape, **kw) as st:
+        st.reset()
+        for t in times:
+            st.update(t, view)
+            st.set_inversion()
+            want.append(st.trace())
+            st.enqueue_root()
+    with T.State(tape, **kw) as st:
+        st.reset()
+        for k, t in enumerate(times):
+            st.update(t, view)
+            st.set_inversion()
+            st.render_async()
+            if k % 2 == 1:
+                rgba, depth = st.trace_wait()
+                assert np.array_equal(rgba.view(np.uint32), want[k][0].view(np.uint32)), k
+                assert np.array_equal(depth.view(np.uint32), want[k][1].view(np.uint32)), k
+        _, _, ntraces = st.trace_totals()
+    assert ntraces == len(times)
+    assert (want[0][1] >= 0).any() and (want[0][1] < 0).any()
+
+
 def test_frames_outside_the_specialised_domain():
     """The tape kernels leave out fix where it is the identity for cells in [-5,5]^3 and
     t-windows in [-2, 3]; a frame at t = 4 leaves that domain, the kernels are rebuilt with every
