@@ -27,6 +27,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <iterator>
 #include <vector>
 
 #include "tsi.h"
@@ -257,6 +258,13 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
          "    return I(wmax(x.lo, k.lo), wmin(x.hi, k.hi));\n}\n";
     o += "TSI_DI uint32_t abs_code(Iv x) { return I_CONTAINS_ZERO(x) ? 3u : (x.hi < 0.0f ? 2u : 1u); }\n";
     o += "struct FwdOut { Iv field; uint32_t cost, records; };\n";
+    // Unpruned modules (lean): the saved-operand stack as global memory, so that global loads are
+    // counted one by one (vmcnt) and the next region's loads stay in flight while this region
+    // waits for its own (through a generic pointer they are flat loads, after which only a wait
+    // for all is possible), and each region's loads issued before the region above runs.
+    // Pruned modules keep loads through the generic pointer, issued inside the region blocks.
+    const bool lean = !prune;
+    if (lean) o += "typedef __attribute__((address_space(1))) float2 gfloat2;\n";
     o += "struct RevOut { Iv s0, s1, s2, s3; uint32_t ops; };\n\n";
 
     // common prologue of the pruned functions: uniform region words, this lane's parent modes
@@ -340,8 +348,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
 
     // choices: the MIN/MAX choices go to chbuf (always with pr)
     auto emit_forward = [&](const std::string &fname, bool choices) {
-    o += "template <bool PUSH>\n__device__ __noinline__ FwdOut " + fname + "(Iv v0, Iv v1, Iv v2, Iv v3, float2 *stk, "
-         "uint32_t lane, const uint32_t *prec, uint32_t cap, uint32_t *chbuf, RegionMask mask) {\n";
+    o += "template <bool PUSH>\n__device__ __noinline__ FwdOut " + fname + "(Iv v0, Iv v1, Iv v2, Iv v3, float2 *" +
+         (lean ? "stk_" : "stk") + ", uint32_t lane, const uint32_t *prec, uint32_t cap, uint32_t *chbuf, RegionMask mask) {\n";
+    if (lean) o += "    gfloat2 *stk = (gfloat2 *)stk_;\n";
     o += "    const Iv ZI = I_FROM(0.0f);\n    const FixK K = fix_consts();\n    uint32_t cost = 0, recs = 0;\n";
     o += prologue(true);
     if (choices)   // skipped regions leave their MIN/MAX undecided (3)
@@ -419,8 +428,9 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // Temporaries of record j carry the suffix _j.  Decided MIN/MAX pass the contribution to
     // the kept operand only (tsi_interp.h rev_step).
     auto emit_reverse = [&](const std::string &fname) {
-        o += "__device__ __noinline__ RevOut " + fname + "(const float2 *stk, uint32_t lane, const uint32_t *prec, "
-             "uint32_t cap, RegionMask mask) {\n";
+        o += "__device__ __noinline__ RevOut " + fname + "(const float2 *" + (lean ? "stk_" : "stk") +
+             ", uint32_t lane, const uint32_t *prec, uint32_t cap, RegionMask mask) {\n";
+        if (lean) o += "    const gfloat2 *stk = (const gfloat2 *)stk_;\n";
         o += "    const Iv ZI = I_FROM(0.0f), ONE = I_FROM(1.0f);\n    const FixK K = fix_consts();\n    uint32_t ops = 0;\n";
         o += prologue(true);
         std::vector<char> touched(n, 0), live(n, 0);
@@ -460,46 +470,90 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 return "(" + bit(region_of(uint32_t(k))) + " ? " + v + " : ZI)";
             return v;
         };
-        auto emit_loads = [&](uint32_t hi, uint32_t lo) {   // saved operands of records hi-1 .. lo
-            std::vector<char> vdone(n, 0), wdone(code_words + 1, 0);
+        // saved values (sv) and code words (kr) read by records hi-1 .. lo
+        auto region_needs = [&](uint32_t hi, uint32_t lo, std::vector<char> &vneed, std::vector<char> &wneed) {
+            vneed.assign(n, 0);
+            wneed.assign(code_words + 1, 0);
             for (uint32_t jj = hi; jj-- > lo;) {
                 if (side_of(tape[jj]) == 3) continue;
                 if (cidx[jj] >= 0) {
-                    const uint32_t w = uint32_t(cidx[jj]) / 16u;
-                    if (!wdone[w])
-                        o += "        kr" + std::to_string(w) + " = __float_as_uint(" + P(nslots + w) + ".x);\n";
-                    wdone[w] = 1;
+                    wneed[uint32_t(cidx[jj]) / 16u] = 1;
                     continue;
                 }
                 for (bool second : {false, true}) {
                     int k = operand_of(jj, second);
                     if (k >= 0 && remat[k] >= 0) k = remat[k];
-                    if (k >= 0 && !vdone[k]) o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
-                    if (k >= 0) vdone[k] = 1;
+                    if (k >= 0) vneed[k] = 1;
                 }
             }
         };
+        // loads of what records hi-1 .. lo read, except what the region running now reads too
+        // (skip): that is in registers already (the stack is not written during the reverse
+        // pass), and reloading it here would make this region wait for the new load
+        auto emit_loads = [&](uint32_t hi, uint32_t lo, const std::vector<char> *vskip = nullptr,
+                              const std::vector<char> *wskip = nullptr) {
+            if (!lean) {   // pruned modules: in record order, each value once per call
+                std::vector<char> vdone(n, 0), wdone(code_words + 1, 0);
+                for (uint32_t jj = hi; jj-- > lo;) {
+                    if (side_of(tape[jj]) == 3) continue;
+                    if (cidx[jj] >= 0) {
+                        const uint32_t w = uint32_t(cidx[jj]) / 16u;
+                        if (!wdone[w])
+                            o += "        kr" + std::to_string(w) + " = __float_as_uint(" + P(nslots + w) + ".x);\n";
+                        wdone[w] = 1;
+                        continue;
+                    }
+                    for (bool second : {false, true}) {
+                        int k = operand_of(jj, second);
+                        if (k >= 0 && remat[k] >= 0) k = remat[k];
+                        if (k >= 0 && !vdone[k]) o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
+                        if (k >= 0) vdone[k] = 1;
+                    }
+                }
+                return;
+            }
+            std::vector<char> vneed, wneed;
+            region_needs(hi, lo, vneed, wneed);
+            for (uint32_t w = 0; w <= code_words; ++w)
+                if (wneed[w] && !(wskip && (*wskip)[w]))
+                    o += "        kr" + std::to_string(w) + " = __float_as_uint(" + P(nslots + w) + ".x);\n";
+            for (uint32_t k = n; k-- > 0;)
+                if (vneed[k] && !(vskip && (*vskip)[k]))
+                    o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
+        };
         // regions from the top: records [hi, lo) with lo the region start
         auto region_lo = [&](uint32_t hi) { return 4u + ((hi - 5u) / G) * G; };
-        if (!pr) {
+        // The loads sit between the region blocks, outside their conditional bodies: assigned
+        // inside one, a value would merge with its old value after the block, and the copy that
+        // merge needs would wait for the load there.  Regions that a pruned program skips get
+        // their operands loaded all the same (the rows exist; the values are then unused).
+        if (lean) emit_loads(n, region_lo(n));   // the top region's operands
+        else if (!pr) {
             o += open_region(0);
             emit_loads(n, region_lo(n));
             o += close_region();
         }
         for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) {
             const uint32_t lo = region_lo(hi), r = region_of(lo);
-            o += open_region(r);
-            if (pr) {
-                // operands of this region: loaded here unless the region above (run before it)
-                // prefetched them
-                if (hi == n) emit_loads(hi, lo);
-                else {
-                    o += "        if (!" + bit(region_of(hi)) + ") {\n";
-                    emit_loads(hi, lo);
-                    o += "        }\n";
-                }
+            if (lean && lo > 4u) {   // the next region's operands go in flight before this region runs
+                std::vector<char> vnow, wnow;
+                region_needs(hi, lo, vnow, wnow);
+                emit_loads(lo, region_lo(lo), &vnow, &wnow);
             }
-            if (lo > 4u) emit_loads(lo, region_lo(lo));   // the next region's operands go in flight
+            o += open_region(r);
+            if (!lean) {
+                if (pr) {
+                    // operands of this region: loaded here unless the region above (run before
+                    // it) prefetched them
+                    if (hi == n) emit_loads(hi, lo);
+                    else {
+                        o += "        if (!" + bit(region_of(hi)) + ") {\n";
+                        emit_loads(hi, lo);
+                        o += "        }\n";
+                    }
+                }
+                if (lo > 4u) emit_loads(lo, region_lo(lo));   // the next region's operands go in flight
+            }
             for (uint32_t j = hi - 1; j >= lo; --j) {
                 const auto &g = tape[j];
                 const FwdIns &f = fwd[j - 4];
@@ -823,6 +877,10 @@ const char *const kOptions[] = {
     "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 };
+// Added for unpruned modules (those that read the stack as global memory): machine sinking would
+// move each region's prefetched loads down into the region that reads them, after the region
+// above, and their latency would be exposed again.
+const char *const kLeanOptions[] = {"-mllvm", "-disable-machine-sink"};
 
 std::string cache_dir() {
     if (const char *d = std::getenv("TSI_JIT_CACHE")) return d;
@@ -873,8 +931,11 @@ int jit_compile(const std::string &src, std::vector<char> &code, std::string &lo
     const auto t0 = std::chrono::steady_clock::now();
     cache_hit = false;
     log.clear();
+    std::vector<const char *> opts(std::begin(kOptions), std::end(kOptions));
+    if (src.find("typedef __attribute__((address_space(1))) float2 gfloat2;") != std::string::npos)
+        opts.insert(opts.end(), std::begin(kLeanOptions), std::end(kLeanOptions));
     std::string key_text = src;
-    for (const char *opt : kOptions) key_text += std::string("\n") + opt;
+    for (const char *opt : opts) key_text += std::string("\n") + opt;
     for (int i = 0; i < tsi_jit_num_headers; ++i) key_text += tsi_jit_header_texts[i];
     int ver_major = 0, ver_minor = 0;
     hiprtcVersion(&ver_major, &ver_minor);
@@ -899,7 +960,7 @@ int jit_compile(const std::string &src, std::vector<char> &code, std::string &lo
             log = "hiprtcCreateProgram failed";
             return TSI_E_JIT;
         }
-        const hiprtcResult rc = hiprtcCompileProgram(prog, int(sizeof kOptions / sizeof kOptions[0]), kOptions);
+        const hiprtcResult rc = hiprtcCompileProgram(prog, int(opts.size()), opts.data());
         size_t ls = 0;
         hiprtcGetProgramLogSize(prog, &ls);
         if (ls > 1) {
