@@ -239,7 +239,7 @@ def main():
     def step():
         """Queues frame k (set inversion, trace, root re-enqueue) and returns frame k-1's result:
         the host never waits for the frame it has just queued, so the device always has the
-        next frame's work; the trace of frame k runs on a second stream beside the set
+        next frame's work; the trace of frame k runs on a trace stream beside the set
         inversion of frame k+1."""
         nonlocal frame, inflight
         st.update(float(np.float32(frame / 60.0)))

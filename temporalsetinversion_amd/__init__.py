@@ -261,7 +261,8 @@ class State:
         return ms.value, visits.value
 
     def trace_async(self):
-        """Starts tracing the current octree (a device snapshot) on a second stream."""
+        """Starts tracing the current octree (a device snapshot) on a trace stream (two trace
+        slots, each with its own stream, snapshot and frame: consecutive traces may overlap)."""
         check(lib().tsi_trace_async(self._ctx), "tsi_trace_async")
 
     def trace_wait(self, pixels: bool = True):
