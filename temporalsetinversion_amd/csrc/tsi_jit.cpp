@@ -57,18 +57,20 @@ std::string lit(float c) {
     return b;
 }
 
-// Records per region (default 16, $TSI_JIT_REGION).  Each region is its own basic block, entered through a uniform branch on
+// Records per region ($TSI_JIT_REGION, else 16 in pruned modules, whose region bits are the
+// pruning unit, and 32 in unpruned ones, where a longer region hides more of the next region's
+// load latency: +2-3 % measured).  Each region is its own basic block, entered through a uniform branch on
 // a value the compiler cannot see through (one scalar instruction), so that instruction
 // selection and scheduling stay local: without it the whole tape is one block in which every
 // load is hoisted to the top and every operation sunk to the bottom.  The reverse pass issues
 // the saved-operand loads of the next region in the current one.
-uint32_t region_records() {
+uint32_t region_records(bool prune) {
     static const uint32_t g = [] {
         const char *e = std::getenv("TSI_JIT_REGION");
         const long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return v >= 4 && v <= 1024 ? uint32_t(v) : 16u;
+        return v >= 4 && v <= 1024 ? uint32_t(v) : 0u;
     }();
-    return g;
+    return g ? g : (prune ? 16u : 32u);
 }
 
 std::string V(uint32_t j) { return "v" + std::to_string(j); }
@@ -197,9 +199,9 @@ std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const st
     return free;
 }
 
-JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L) {
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune) {
     JitLayout y;
-    y.group = region_records();
+    y.group = region_records(prune);
     y.regions = (L.nfwd + y.group - 1) / y.group;
     y.region_words = (y.regions + 31) / 32;
     for (uint32_t k = 0; k < L.nfwd; ++k) y.minmax += is_minmax_form(fwd[k].form) ? 1u : 0u;
@@ -213,7 +215,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                        const JitDomain *domain) {
     std::string o;
     o.reserve(size_t(n) * 500u);
-    const JitLayout Y = jit_layout(fwd, L);
+    const JitLayout Y = jit_layout(fwd, L, prune);
     const uint32_t G = Y.group, RW = Y.region_words, MW = Y.mode_words;
     auto region_of = [&](uint32_t j) { return (j - 4) / G; };
     // MIN/MAX index of record j (records in order)

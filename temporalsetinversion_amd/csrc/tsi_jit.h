@@ -14,7 +14,8 @@ namespace tsi {
 struct JitLayout {
     uint32_t group = 16, regions = 0, region_words = 0, minmax = 0, mode_words = 0, record_words = 0;
 };
-JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L);
+// (prune: the layout of pruned modules, the only ones that exchange records)
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune = true);
 
 // The domain a tape's kernels may be specialised for: every cell box inside `box` (lo xyz, hi
 // xyz), every t-window inside [t0, t1].  fix_free marks the records whose value no cell in it can
