@@ -1,15 +1,18 @@
 """Benchmark: temporal set inversion on one MI355X.
 
-One step = one frame of the plain per-frame path on the 64-capsule workload (seeded
-smooth-min union of 64 capsules translating linearly in t under a twist warp of angle
-2 t z; localsubdiv 9 = 512^3 cells; 1024x1024 frame; delta 0.01; Lipschitz correction on):
-set inversion (tsi_set_inversion) + trace of the frame (tsi_trace) + root re-enqueue,
-at t_k = k/60.  The first W frames are untimed warm-up (frame 0 evaluates every cell);
-the next K frames are timed with the device synchronised on both sides.
+One step = one frame of the plain per-frame path on the 1024^3 workload (seeded straight-line
+CSG of 256 spheres / boxes / tori under min / max / smooth-min with sin/cos domain warps in x, y,
+z and t; localsubdiv 10 = 1024^3 cells; 1920x1080 frame; delta 0.01; swept over each frame's
+window [t - 1/60, t]; point-sampled leaves): set inversion + trace of the frame + root
+re-enqueue, at t_k = k/60.  The first W frames are untimed warm-up (frame 0 evaluates every
+cell); the next K frames are timed with the device synchronised on both sides.
 
 Prints one JSON line.  `value` = interval cell-evals per second over the timed frames;
-`roofline` reports the tape-evaluation kernels against the FP32 VALU peak; `cpu_baseline`
-times the oracle (the C restatement of the reference) on a bounded sample of the same tape.
+`roofline` reports the tape-evaluation kernel against the FP32 VALU peak (its device time from
+HIP events right around each level launch); `temporal_vs_full` compares the set inversion of
+the temporal frames with full re-evaluation (every frame from an empty octree), the quantity
+the reference publishes (~3x at 1024^3, delta 0.01); `cpu_baseline` times the oracle (the C
+restatement of the reference) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -24,124 +27,185 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X FP32 vector issue: 256 CU x 4 SIMD x 16 lanes x 4 (wave64 over 4 cycles) ... expressed as
-# lane-results: 157.3 TFLOP/s counts an FMA as 2 and assumes packed dual-FP32 issue, i.e.
-# 78.6 T f32 results/s; without packed issue 39.3 T/s.
-PEAK_VALU_TOPS = 78.6
+# MI355X FP32 vector peak as published: 157.3 TFLOP/s (256 CUs x 2.4 GHz x 256 flops per clock:
+# an FMA counted as 2 and packed dual-FP32 issue).  The tape kernels count one op per f32 result
+# and cannot use FMA (results must be bit-identical to the unfused rules) or packed FP32 (off on
+# this machine); their attainable rate is one result per lane per clock, 39.3 T/s.
+PEAK_VALU_TFLOPS = 157.3
 PEAK_VALU_UNPACKED_TOPS = 39.3
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
-    # plain method on one MI355X: 512^3, 1024x1024, 64 capsules
+    # plain method on one MI355X: 512^3, 1024x1024, 64 capsules (512^3 cells)
     "capsules64_L9_1024": dict(scene="capsules", seed=1, localsubdiv=9, width=1024, height=1024,
                                lipschitz_corrected=True, swept_volume=False, point_sample=True),
     # plain method, large: 1024^3, 1920x1080, 256-primitive CSG, swept, point-sampled
     "csg256_L10_1920x1080": dict(scene="csg", seed=2, localsubdiv=10, width=1920, height=1080,
                                  lipschitz_corrected=False, swept_volume=True, point_sample=True),
-    # the 8-moving-spheres field at 64^3
+    # the 8-moving-spheres field at 64^3 (the reference's CPU configuration)
     "spheres8_L6_256": dict(scene="spheres", seed=0, localsubdiv=6, width=256, height=256,
                             lipschitz_corrected=False, swept_volume=False, point_sample=True),
+    # the capsules at 1024^3 (an animated scene at the resolution of the reference's published
+    # temporal-vs-full figure)
+    "capsules64_L10_1024": dict(scene="capsules", seed=1, localsubdiv=10, width=1024, height=1024,
+                                lipschitz_corrected=True, swept_volume=False, point_sample=True),
 }
+DEFAULT_WORKLOAD = "csg256_L10_1920x1080"
+
+
+def _profile_path(workload: str) -> str:
+    return os.path.join(ROOT, "profiles", f"tape_eval_{workload}.txt")
 
 
 def profiled_traffic(workload: str, engine: str):
-    """HBM bytes per eval launch from the committed rocprofv3 summary of this workload, if any
-    (FETCH_SIZE/WRITE_SIZE passes scaled by the copy calibration; see profiles/)."""
-    path = os.path.join(ROOT, "profiles", f"tape_eval_{workload.split('_')[0]}_{workload.split('_', 1)[1]}.txt")
+    """(HBM bytes per eval launch, summary path, average kernel us) from the committed rocprofv3
+    summary of this workload (timed frames only), if any."""
+    path = _profile_path(workload)
     if engine != "compiled" or not os.path.exists(path):
         return None, None, None
-    rows, times = {}, {}
-    in_traffic = False
+    traffic = avg = None
     for line in open(path):
         parts = line.split()
-        in_traffic = in_traffic or "hbm_bytes/disp" in line
-        if in_traffic and len(parts) == 5 and parts[0].startswith("tsi_tape_eval_"):
-            rows[parts[0]] = (int(parts[1]), float(parts[4]))
-        elif not in_traffic and len(parts) == 5 and parts[0].startswith("tsi_tape_eval_"):
-            times[parts[0]] = (int(parts[1]), float(parts[3]))   # dispatches, avg_us
-    if not rows:
-        return None, None, None
-    n = sum(k for k, _ in rows.values())
-    m = sum(k for k, _ in times.values())
-    avg_us = sum(k * t for k, t in times.values()) / m if m else None
-    return sum(k * b for k, b in rows.values()) / n, os.path.relpath(path, ROOT), avg_us
+        if len(parts) >= 5 and parts[0] == "tsi_tape_eval_level" and parts[1] == "timed":
+            avg = float(parts[4])
+        if len(parts) >= 5 and parts[0] == "tsi_tape_eval_level" and parts[1] == "hbm":
+            traffic = float(parts[4])
+    return traffic, os.path.relpath(path, ROOT), avg
 
 
-def _short(name: str) -> str:
-    return name.split("(")[0]
+def _csv_rows(path):
+    import csv
+
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _kernel(name: str) -> str:
+    return name.split("(")[0].strip()
 
 
 def summarize_profile(args) -> None:
-    """Prints the committed profile summary (profiles/tape_eval_<workload>.txt) from the
-    rocprofv3 databases of the kernel-trace run, the FETCH_SIZE / WRITE_SIZE runs of the same
-    workload and the same two counters over a 256 MiB device copy (the scale that turns the
-    counters into bytes on this machine)."""
-    import sqlite3
-
-    def per_kernel(db, sql):
-        out = {}
-        for name, n, v in sqlite3.connect(db).execute(sql):
-            k = _short(name)
-            m, t = out.get(k, (0, 0.0))
-            out[k] = (m + n, t + v)
-        return out
-
-    kt = per_kernel(args.summarize, "select name, count(*), sum(duration) from kernels group by name")
-    total = sum(t for _, t in kt.values())
-    frames = kt.get("tsi::k_frame_begin", (0, 0))[0]
-    W, R = args.workload, "$R"
-    tag = {"fetch": args.pmc_fetch, "write": args.pmc_write}
-    lines = [f"# Kernel time and HBM traffic: {W}, compiled tape engine, bench defaults",
+    """Prints the committed profile summary (profiles/tape_eval_<workload>.txt) from rocprofv3 CSV
+    output: the kernel trace of a bench run, the counter-collection runs of the same command
+    (FETCH_SIZE, WRITE_SIZE, SQ counters) and FETCH_SIZE / WRITE_SIZE over a 256 MiB device copy
+    (the scale that turns those counters into bytes on this machine).  Dispatches of the first
+    --skip-frames frames (the warm-up: frame 0 evaluates every cell) are left out of the averages."""
+    W, skip = args.workload, args.skip_frames
+    L = WORKLOADS[W]["localsubdiv"]
+    rows = sorted(_csv_rows(args.summarize), key=lambda r: int(r["Start_Timestamp"]))
+    # frames are delimited by k_frame_begin
+    frame_of, k = [], -1
+    for r in rows:
+        if _kernel(r["Kernel_Name"]) == "tsi::k_frame_begin":
+            k += 1
+        frame_of.append(k)
+    frames = k + 1
+    per, timed = {}, {}
+    for r, f in zip(rows, frame_of):
+        name = _kernel(r["Kernel_Name"])
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3   # us
+        n, t = per.get(name, (0, 0.0))
+        per[name] = (n + 1, t + d)
+        if f >= skip:
+            n, t = timed.get(name, (0, 0.0))
+            timed[name] = (n + 1, t + d)
+    total = sum(t for _, t in per.values())
+    R = "$R"
+    lines = [f"# Kernel time, HBM traffic and SQ counters: {W}, compiled tape engine, bench defaults",
              "#",
-             "# commands (run from /tmp on the GPU host, repo at $R; <d> = an output directory):",
-             f"#   rocprofv3 --kernel-trace --stats -d <d>/kt -o kt -- python3 {R}/bench.py --steps 5 --warmup 2 --no-cpu --workload {W}"]
+             "# commands (from /tmp on the GPU host, repo at $R, <d> = an output directory):",
+             f"#   rocprofv3 --kernel-trace --stats -f csv -d <d>/kt -o kt -- python3 {R}/bench.py --steps 5 --warmup 2 "
+             f"--no-cpu --no-full --workload {W}"]
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
-        lines.append(f"#   rocprofv3 --pmc {c} -d <d>/{c} -o pmc -- python3 {R}/bench.py --steps 2 --warmup 1 --no-cpu --workload {W}")
-        lines.append(f"#   rocprofv3 --pmc {c} -d <d>/copy_{c} -o pmc -- python3 {R}/bench.py --calibrate-copy-mib {args.copy_mib}")
-    lines.append(f"#   python3 bench.py --workload {W} --summarize <d>/kt/kt_results.db --pmc-fetch <d>/FETCH_SIZE/pmc_results.db "
-                 "--pmc-write <d>/WRITE_SIZE/pmc_results.db --copy-fetch <d>/copy_FETCH_SIZE/pmc_results.db "
-                 "--copy-write <d>/copy_WRITE_SIZE/pmc_results.db")
-    lines += ["#", f"# {frames} frames (warm-up + timed); one level kernel per octree level running the forward-only",
-              "# and the reverse list side by side (tsi_tape_eval_level), one snapshot + trace per frame on",
-              "# a trace stream.", ""]
-    lines.append(f"{'kernel':<44} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
-    for k, (n, t) in sorted(kt.items(), key=lambda x: -x[1][1]):
-        lines.append(f"{k[:44]:<44} {n:>10} {t / 1e6:>10.3f} {t / n / 1e3:>9.2f} {100 * t / total:>6.1f}%")
+        lines.append(f"#   rocprofv3 --pmc {c} -f csv -d <d>/{c} -o pmc -- python3 {R}/bench.py --steps 2 --warmup 1 "
+                     f"--no-cpu --no-full --workload {W}")
+        lines.append(f"#   rocprofv3 --pmc {c} -f csv -d <d>/copy_{c} -o pmc -- python3 {R}/bench.py --calibrate-copy-mib "
+                     f"{args.copy_mib}")
+    if args.pmc_sq:
+        lines.append(f"#   rocprofv3 --pmc <SQ counters of each pass below> -f csv -d <d>/SQ<n> -o pmc -- python3 {R}/bench.py "
+                     f"--steps 2 --warmup 1 --no-cpu --no-full --workload {W}")
+    lines.append(f"#   python3 bench.py --workload {W} --skip-frames {skip} --summarize <d>/kt/..._kernel_trace.csv "
+                 "--pmc-fetch <FETCH_SIZE counter_collection.csv> --pmc-write <WRITE_SIZE ...> --copy-fetch <...> "
+                 "--copy-write <...> --pmc-sq <SQ pass csv files>")
+    lines += ["#", f"# {frames} frames ({skip} warm-up, {frames - skip} timed); one tape-evaluation launch per octree "
+              f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, one snapshot + "
+              "trace per frame on a trace stream.", ""]
+    lines.append(f"{'kernel':<40} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
+    for name, (n, t) in sorted(per.items(), key=lambda x: -x[1][1]):
+        lines.append(f"{name[:40]:<40} {n:>10} {t / 1e3:>10.3f} {t / n:>9.2f} {100 * t / total:>6.1f}%")
+    lines.append("")
+    lines.append("timed frames only (the figures bench.py reads):")
+    for name in ("tsi_tape_eval_level", "tsi::k_trace"):
+        if name in timed:
+            n, t = timed[name]
+            lines.append(f"{name} timed {n} dispatches avg_us {t / n:.2f}")
     if args.pmc_fetch and args.pmc_write and args.copy_fetch and args.copy_write:
-        q = "select name, count(*), sum(counter_value) from pmc_events group by name"
+        def per_kernel(path, counter):
+            """{kernel: [values per dispatch in dispatch order]} of one counter."""
+            out = {}
+            for r in sorted(_csv_rows(path), key=lambda r: int(r["Dispatch_Id"])):
+                if r["Counter_Name"] == counter:
+                    out.setdefault(_kernel(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+            return out
+
         kib = args.copy_mib * 1024.0
 
-        def scale(db):   # KiB the counter reports per copy dispatch -> bytes
-            n, v = per_kernel(db, q)["__amd_rocclr_copyBuffer"]
-            return kib / (v / n)
+        def scale(path, counter):   # KiB the counter reports per copy dispatch -> bytes
+            v = [x for k, xs in per_kernel(path, counter).items() if "copyBuffer" in k for x in xs]
+            return kib / (sum(v) / len(v))
 
-        sf, sw = scale(args.copy_fetch), scale(args.copy_write)
-        f, w = per_kernel(args.pmc_fetch, q), per_kernel(args.pmc_write, q)
+        sf, sw = scale(args.copy_fetch, "FETCH_SIZE"), scale(args.copy_write, "WRITE_SIZE")
+        f, w = per_kernel(args.pmc_fetch, "FETCH_SIZE"), per_kernel(args.pmc_write, "WRITE_SIZE")
+        # the counter runs have 1 warm-up frame: its L level launches are left out
         lines += ["", f"copy calibration ({args.copy_mib} MiB device-to-device): FETCH_SIZE scale {sf:.3f}, "
-                      f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024)",
-                  f"{'kernel':<24} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'hbm_bytes/disp':>15}"]
+                      f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024); level launches of the timed frames",
+                  f"{'kernel':<24} {'':>4} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'bytes/disp':>15}"]
         for k in ("tsi_tape_eval_level", "tsi::k_trace", "tsi::k_snapshot"):
             if k in f and k in w:
-                nf, vf = f[k]
-                nw, vw = w[k]
-                b = (vf / nf * sf + vw / nw * sw) * 1024.0
-                lines.append(f"{k:<24} {nf:>10} {vf / nf:>15.1f} {vw / nw:>15.1f} {b:>15.0f}")
+                cut = L if k == "tsi_tape_eval_level" else 1
+                vf, vw = f[k][cut:], w[k][cut:]
+                if not vf or not vw:
+                    continue
+                b = (sum(vf) / len(vf) * sf + sum(vw) / len(vw) * sw) * 1024.0
+                lines.append(f"{k:<24} {'hbm':>4} {len(vf):>10} {sum(vf) / len(vf):>15.1f} {sum(vw) / len(vw):>15.1f} "
+                             f"{b:>15.0f}")
+    for path in args.pmc_sq:
+        rows = _csv_rows(path)
+        counters = sorted({r["Counter_Name"] for r in rows})
+        vals = {}
+        for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+            if _kernel(r["Kernel_Name"]) == "tsi_tape_eval_level":
+                vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        lines += ["", f"SQ counters per tsi_tape_eval_level dispatch (timed frames), {os.path.basename(path)}:"]
+        for c in counters:
+            v = vals.get(c, [])[L:]
+            if v:
+                lines.append(f"  {c:<28} {sum(v) / len(v):>16.1f}")
     print("\n".join(lines))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _cpu_worker(job):
-    """One oracle frame sequence (frames t_k = (k + phase)/60) until the time budget is spent."""
+    """Oracle frames t_k = k/60 over one sub-box of the root until the time budget is spent."""
     import oracle
 
-    ll, w, L, phase, budget_s = job
+    ll, w, L, box, budget_s = job
     tape = oracle.parse_ll(ll)
-    fr = oracle.Frame(tape, localsubdiv=L, swept_volume=w["swept_volume"],
-                      lipschitz_corrected=w["lipschitz_corrected"], point_sample=w["point_sample"],
-                      voxel_capacity=1 << 22)
+    fr = oracle.Frame(tape, localsubdiv=L, swept_volume=w["swept_volume"], lipschitz_corrected=w["lipschitz_corrected"],
+                      point_sample=w["point_sample"], voxel_capacity=1 << 22, root_box=box)
     cells = frames = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s and frames < 240:
-        st = fr.set_inversion(float(np.float32((frames + phase) / 60.0)))
+        st = fr.set_inversion(float(np.float32(frames / 60.0)))
         cells += st.cell_evals
         frames += 1
         fr.enqueue_root()
@@ -150,24 +214,121 @@ def _cpu_worker(job):
     return cells, frames, dt
 
 
-def cpu_baseline(ll, w, budget_s: float):
-    """The oracle (C restatement of compute.wgsl `main`) on the host cores: one independent frame
-    sequence per worker process (phases 7 frames apart), same tape and parameters at localsubdiv
-    L-3, each until the time budget is spent.  Runs before the GPU is initialised."""
+def _sub_boxes(ll, w, depth, count, seed=0):
+    """Boxes of `count` depth-`depth` octree cells the surface passes through (seeded choice),
+    from the oracle's first frame at that depth."""
+    import oracle
+
+    fr = oracle.Frame(oracle.parse_ll(ll), localsubdiv=depth, swept_volume=w["swept_volume"],
+                      lipschitz_corrected=w["lipschitz_corrected"], point_sample=False, voxel_capacity=1 << 20)
+    fr.set_inversion(0.0)
+    n = fr.export()
+    fr.close()
+    cand = n[(n["depth"] == depth) & (n["f"] == 0.0)]
+    pick = cand[np.random.default_rng(seed).choice(len(cand), min(count, len(cand)), replace=False)]
+    size = 10.0 / (1 << depth)
+    out = []
+    for c in pick:
+        lo = [-5.0 + size * int(c[a]) for a in ("i", "j", "k")]
+        out.append(tuple(lo) + tuple(x + size for x in lo))
+    return out
+
+
+def cpu_baseline(ll, w, budget_s: float, workers: int):
+    """The oracle (C restatement of compute.wgsl `main`) on the host cores over a bounded sample of
+    the same workload: `workers` of the 512 depth-3 subtrees of the octree that the surface passes
+    through, one per worker process, each an octree of its own over that sub-box with the same
+    tape, parameters and epsilon (so the same cells as that subtree of the full octree), frames
+    t_k = k/60 until the time budget is spent.  Runs before the GPU is initialised."""
     import multiprocessing as mp
 
-    L = max(3, w["localsubdiv"] - 3)
-    workers = max(1, min(16, os.cpu_count() or 1))
+    boxes = _sub_boxes(ll, w, 3, workers)
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
-    with ctx.Pool(workers) as pool:
-        res = pool.map(_cpu_worker, [(ll, w, L, 7 * i, budget_s) for i in range(workers)])
+    with ctx.Pool(len(boxes)) as pool:
+        res = pool.map(_cpu_worker, [(ll, w, w["localsubdiv"], b, budget_s) for b in boxes])
     wall = time.perf_counter() - t0
     cells = sum(r[0] for r in res)
     frames = sum(r[1] for r in res)
-    return {"value": cells / wall, "unit": "cell-evals/s", "cores": workers, "kind": "port",
-            "sample": f"oracle (C restatement) on {workers} host cores, one frame sequence per core, same tape "
-                      f"and parameters at localsubdiv {L}, {frames} frames, {cells} cell-evals in {wall:.1f} s"}
+    return {"value": cells / wall, "unit": "cell-evals/s", "cores": len(boxes), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle (CPU restatement, not lavapipe: no Vulkan driver or cargo on the host) on "
+                      f"{len(boxes)} host cores (the host's CPU share for one GPU), one process per depth-3 "
+                      f"subtree the surface passes through ({len(boxes)} of 512), same tape, localsubdiv, "
+                      f"delta, swept window and point sampling; {frames} frames, {cells} cell-evals in {wall:.1f} s"}
+
+
+def cpu_baseline_c1(budget_s: float, workers: int):
+    """The reference's CPU configuration (8 moving spheres, localsubdiv 6, 256x256 frame): one frame
+    sequence per worker (phases 7 frames apart), set inversion and trace per frame."""
+    import multiprocessing as mp
+
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_c1_worker, [(7 * i, budget_s) for i in range(workers)])
+    wall = time.perf_counter() - t0
+    frames = sum(r[0] for r in res)
+    cells = sum(r[1] for r in res)
+    return {"frames_per_s": frames / wall, "cell_evals_per_s": cells / wall, "cores": workers, "kind": "port",
+            "config": "spheres8_L6_256 (8 spheres, c(t) = c0 + 0.2 sin(2 pi t + phi), localsubdiv 6, 256x256, "
+                      "delta 0.01, t_k = k/60)", "frames": frames, "seconds": wall}
+
+
+def _c1_worker(job):
+    import oracle
+    import temporalsetinversion_amd as T
+    from temporalsetinversion_amd import scenes
+
+    phase, budget_s = job
+    fr = oracle.Frame(oracle.parse_ll(scenes.spheres(0)), localsubdiv=6, voxel_capacity=1 << 20)
+    view = T.diagonal_camera()
+    zd = float(np.float32(256.0) / np.float32(np.tan(np.float32(np.pi / 4))))   # compute.wgsl:4592
+    frames = cells = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and frames < 600:
+        t = float(np.float32((frames + phase) / 60.0))
+        cells += fr.set_inversion(t).cell_evals
+        fr.trace(view, 256, 256, t, zd)
+        fr.enqueue_root()
+        frames += 1
+    fr.close()
+    return frames, cells
+
+
+def full_reevaluation(st, frames: int, t0_frame: int):
+    """Set inversion of `frames` frames each from an empty octree (tsi_reset before the frame: every
+    cell new), device ms per frame; the temporal method's comparison of the reference."""
+    ms = cells = 0.0
+    for k in range(frames):
+        st.update(float(np.float32((t0_frame + k) / 60.0)))
+        st.reset()
+        r = st.set_inversion()
+        ms += r.gpu_ms
+        cells += r.stats["cell_evals"]
+    return ms / frames, cells / frames
+
+
+def temporal_frames(st, frames: int, warmup: int):
+    """Set inversion of `frames` consecutive temporal frames after `warmup` untimed ones, from a
+    reset; device ms and cell-evals per frame."""
+    st.reset()
+    ms = cells = 0.0
+    for k in range(warmup + frames):
+        st.update(float(np.float32(k / 60.0)))
+        r = st.set_inversion()
+        st.enqueue_root()
+        if k >= warmup:
+            ms += r.gpu_ms
+            cells += r.stats["cell_evals"]
+    return ms / frames, cells / frames
+
+
+def make_state(T, w, tape, prune, engine):
+    return T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
+                   point_sample=w["point_sample"], swept_volume=w["swept_volume"],
+                   lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27,
+                   prune={"on": True, "off": False, "exact": "exact"}[prune], engine=engine)
 
 
 def main():
@@ -175,9 +336,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="capsules64_L9_1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline worker")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full", action="store_true", help="skip the temporal-vs-full re-evaluation comparison")
     ap.add_argument("--engine", default="compiled", choices=["compiled", "interpreter"],
                     help="tape evaluation: compiled straight-line kernels (default) or the interpreter")
     ap.add_argument("--sync-trace", action="store_true",
@@ -193,8 +355,11 @@ def main():
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--summarize", metavar="KERNEL_TRACE_DB",
                     help="instead of the benchmark: summarise a rocprofv3 --kernel-trace --stats database")
+    ap.add_argument("--skip-frames", type=int, default=0,
+                    help="with --summarize: leave out the first N frames' dispatches (the warm-up frames)")
     ap.add_argument("--pmc-fetch", metavar="DB")
     ap.add_argument("--pmc-write", metavar="DB")
+    ap.add_argument("--pmc-sq", metavar="DB", nargs="*", default=[])
     ap.add_argument("--copy-fetch", metavar="DB")
     ap.add_argument("--copy-write", metavar="DB")
     ap.add_argument("--copy-mib", type=int, default=256)
@@ -222,16 +387,19 @@ def main():
 
     w = WORKLOADS[args.workload]
     ll = scenes.SCENES[w["scene"]](w["seed"])
-    cpu = None if args.no_cpu else cpu_baseline(ll, w, args.cpu_budget)
+    cpu = cpu_c1 = None
+    if not args.no_cpu:   # before the GPU is touched (worker processes are forked)
+        workers = max(1, min(16, os.cpu_count() or 1))
+        print(f"cpu baseline: {workers} workers x {args.cpu_budget:.0f} s (same workload), then the spheres "
+              f"configuration", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(ll, w, args.cpu_budget, workers)
+        cpu_c1 = cpu_baseline_c1(min(args.cpu_budget, 6.0), workers)
 
     import temporalsetinversion_amd as T
 
     tape = T.tape_from_ll(ll)
     prune = args.prune if args.prune != "auto" else "off"
-    st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
-                 point_sample=w["point_sample"], swept_volume=w["swept_volume"],
-                 lipschitz_corrected=w["lipschitz_corrected"], voxel_capacity=1 << 27,
-                 prune={"on": True, "off": False, "exact": "exact"}[prune], engine=args.engine)
+    st = make_state(T, w, tape, prune, args.engine)
     info = st.tape_info()
     st.reset()
     frame = 0
@@ -265,13 +433,14 @@ def main():
         return out
 
     inflight = 0
+    print(f"{args.workload}: {args.warmup} warm-up + {args.steps} timed frames", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
     drain()
     st.trace_wait(pixels=False)
     frame_results = []
     ms0, visits0, n0 = st.trace_totals()
-    tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, launches=0, pruned=0)
+    tot = dict(cells=0, rev=0, fwd_ops=0, rev_ops=0, gpu_ms=0.0, eval_ms=0.0, kernel_ms=0.0, launches=0, pruned=0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         frame_results += step()
@@ -286,16 +455,37 @@ def main():
         tot["rev_ops"] += s["rev_ops"]
         tot["gpu_ms"] += s["gpu_ms"]
         tot["eval_ms"] += s["eval_ms"]
+        tot["kernel_ms"] += s["eval_kernel_ms"]
         tot["launches"] += s["eval_launches"]
         tot["pruned"] += s["pruned_instr"]
     assert len(frame_results) == args.steps
     ms1, visits1, n1 = st.trace_totals()
     assert n1 - n0 == args.steps
     trace = dict(ms=ms1 - ms0, visits=visits1 - visits0)
+    print(f"timed: {wall * 1e3 / args.steps:.3f} ms per frame", file=sys.stderr, flush=True)
+
+    # the reference's published comparison: temporal frames against full re-evaluation
+    tvf = None
+    if not args.no_full:
+        si_ms = tot["gpu_ms"] / args.steps
+        full_ms, full_cells = full_reevaluation(st, 5, frame)
+        tvf = {"workload": args.workload, "temporal_set_inversion_ms": si_ms, "full_set_inversion_ms": full_ms,
+               "speedup": full_ms / si_ms, "temporal_cell_evals": tot["cells"] / args.steps,
+               "full_cell_evals": full_cells}
+        if args.workload != "capsules64_L10_1024":
+            w2 = WORKLOADS["capsules64_L10_1024"]
+            st2 = make_state(T, w2, T.tape_from_ll(scenes.SCENES[w2["scene"]](w2["seed"])), "off", args.engine)
+            t_ms, t_cells = temporal_frames(st2, 10, 3)
+            f_ms, f_cells = full_reevaluation(st2, 5, 13)
+            st2.close()
+            tvf["capsules64_L10_1024"] = {"temporal_set_inversion_ms": t_ms, "full_set_inversion_ms": f_ms,
+                                          "speedup": f_ms / t_ms, "temporal_cell_evals": t_cells,
+                                          "full_cell_evals": f_cells}
     st.close()
 
     ops = tot["fwd_ops"] + tot["rev_ops"]
-    achieved = ops / (tot["eval_ms"] * 1e-3) / 1e12 if tot["eval_ms"] > 0 else 0.0
+    kernel_s = tot["kernel_ms"] * 1e-3
+    achieved = ops / kernel_s / 1e12 if kernel_s > 0 else 0.0
     launches = max(tot["launches"], 1)
     traffic, traffic_src, prof_avg_us = profiled_traffic(args.workload, info["engine"])
     # trace: SURVEY-style algorithmic bytes, 40 B per node visit + 20 B per pixel (colour + depth)
@@ -331,24 +521,30 @@ def main():
                      "kernel": ("tsi_tape_eval_level (one launch per octree level: forward-only and reverse lists)"
                                 if info["engine"] == "compiled" else
                                 "k_eval<false>/<true> (one forward-only and one reverse launch per octree level)"),
-                     "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "T f32-ops/s",
-                     "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic, "traffic_unit": "HBM bytes per launch",
-                     "traffic_source": traffic_src,
+                     "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_VALU_TFLOPS,
+                     "frac_vs_unpacked_issue": achieved / PEAK_VALU_UNPACKED_TOPS,
+                     "peak_unpacked_issue": PEAK_VALU_UNPACKED_TOPS,
+                     "traffic": traffic, "traffic_unit": "HBM bytes per launch", "traffic_source": traffic_src,
                      "algorithmic_ops_per_launch": ops / launches,
-                     # HIP events around the level loop: kernels plus the dispatch gaps between them
-                     "avg_launch_us": tot["eval_ms"] * 1e3 / launches,
-                     # the same kernels' average duration in the committed rocprofv3 summary
+                     "ops_per_cell_eval": ops / max(tot["cells"], 1),
+                     # HIP events right before and after each level launch, over the timed frames
+                     "avg_launch_us": tot["kernel_ms"] * 1e3 / launches,
+                     # the same kernel's average over the timed frames of the committed rocprofv3 summary
                      "profile_avg_kernel_us": prof_avg_us,
-                     "peak_unpacked": PEAK_VALU_UNPACKED_TOPS,
-                     "eval_ms_per_frame": tot["eval_ms"] / args.steps,
+                     "eval_ms_per_frame": tot["kernel_ms"] / args.steps,
+                     "eval_span_ms_per_frame": tot["eval_ms"] / args.steps,
                      "ops_per_frame": ops / args.steps},
         "roofline_trace": {"bound": "hbm", "kernel": "k_trace", "achieved": trace_gbs, "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": trace_gbs / PEAK_HBM_GBS,
                            "trace_ms_per_frame": trace["ms"] / args.steps,
                            "node_visits_per_frame": trace["visits"] / args.steps},
     }
+    if tvf is not None:
+        out["temporal_vs_full"] = tvf
     if cpu is not None:
         out["cpu_baseline"] = cpu
+        out["cpu_baseline_c1"] = cpu_c1
     print(json.dumps(out), flush=True)
 
 

@@ -112,6 +112,9 @@ typedef struct {
     uint64_t eval_launches;       /* tape-evaluation kernel launches */
     uint64_t pruned_instr;        /* sum over evaluated cells of instructions skipped by pruning */
     uint64_t prune_launches;      /* group-program generator launches */
+    double eval_kernel_ms;        /* the evaluation launches alone: device time between events
+                                     recorded right before and after each level's launches
+                                     (eval_ms also holds the gaps between levels) */
 } tsi_stats;
 
 /* Canonical node record (see tsi_read_nodes). */
@@ -224,6 +227,13 @@ int tsi_tape_info_get(tsi_ctx *ctx, tsi_tape_info *out);
  * same pruning flags (TSI_FLAG_NO_PRUNE, TSI_FLAG_PRUNE_EXACT_ONLY) then only loads them.  Cache directory: $TSI_JIT_CACHE, else $XDG_CACHE_HOME/tsi-jit, else
  * ~/.cache/tsi-jit. */
 int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t flags, tsi_tape_info *out);
+
+/* The source of the evaluation kernels tsi_precompile_tape / tsi_upload_tape would compile for a
+ * tape under these flags, without compiling it (no counterpart in the reference, which interprets
+ * the tape): writes min(cap - 1, length) characters and a NUL to out (may be NULL); *len receives
+ * the length; *info (may be NULL) the lowering's figures (n, stack, slots, prune_class). */
+int tsi_tape_kernel_source(const tsi_gpu_instruction *tape, uint32_t n, uint32_t flags, char *out, size_t cap,
+                           size_t *len, tsi_tape_info *info);
 
 /* src/ir.rs IR::new + parseLLVMIR + to_gpu: builds the tape from .ll text holding
  * @_Z1fffff.  time is written into instruction 3 (ARGW).  *n receives the full

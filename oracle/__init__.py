@@ -101,6 +101,12 @@ def lib():
         L.orc_trace.restype = C.c_int
         L.orc_export.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
         L.orc_export.restype = C.c_uint64
+        L.orc_leafset_build.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.c_double]
+        L.orc_leafset_build.restype = C.c_void_p
+        L.orc_leafset_free.argtypes = [C.c_void_p]
+        L.orc_leafset_free.restype = None
+        L.orc_leafset_march.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.orc_leafset_march.restype = None
         _lib = L
     return _lib
 
@@ -202,3 +208,36 @@ class Frame:
             self.close()
         except Exception:
             pass
+
+
+def pixel_rays(view, width: int, height: int, xs, ys):
+    """float64 eye and unit directions of pixels (xs, ys): ray_direction (compute.wgsl:4590-4594,
+    fov 90: z = height / tan(45 degrees) = height) turned by the view matrix (4630)."""
+    m = np.asarray(view, np.float64).reshape(4, 4)   # column-major: m[c] is column c
+    xs = np.asarray(xs, np.float64)
+    ys = np.asarray(ys, np.float64)
+    d = np.stack([(xs - width / 2.0) + 0.5, -((ys - height / 2.0) + 0.5), -np.full_like(xs, float(height))], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    g = d @ m[:3, :3]
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    eye = np.broadcast_to(m[3, :3], g.shape).copy()
+    return eye, g
+
+
+def march_depth(nodes: np.ndarray, origins: np.ndarray, dirs: np.ndarray,
+                root_box=(-5.0, -5.0, -5.0, 5.0, 5.0, 5.0), pad: float = 1e-4) -> np.ndarray:
+    """float64 first-hit distance of each ray against a canonical node export (march.c), -1 on a
+    miss: the independent restatement of the depth the trace reports."""
+    nd = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    o = np.ascontiguousarray(origins, np.float64)
+    d = np.ascontiguousarray(dirs, np.float64)
+    box = (C.c_double * 6)(*root_box)
+    h = lib().orc_leafset_build(nd.ctypes.data, len(nd), box, pad)
+    if not h:
+        raise MemoryError("orc_leafset_build failed")
+    out = np.zeros(len(o), np.float64)
+    try:
+        lib().orc_leafset_march(h, o.ctypes.data, d.ctypes.data, len(o), out.ctypes.data)
+    finally:
+        lib().orc_leafset_free(h)
+    return out

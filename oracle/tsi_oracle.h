@@ -147,6 +147,17 @@ typedef struct {
 uint64_t orc_export(const orc_state *s, orc_node *out, uint64_t cap);
 const orc_voxel *orc_voxels(const orc_state *s, uint32_t *n_used);
 
+/* ---- float64 ray march of a leaf set (march.c): the independent depth check ---- */
+/* nodes: a canonical export (orc_export / tsi_read_nodes), borrowed while the set lives;
+ * boxes are root_box cut into 2^depth cells per axis, children widened by pad (1e-4 in the
+ * reference's trace, compute.wgsl:4310-4311).  depth_out[r] = distance along the unit ray r
+ * to the first entered leaf with f <= 0 reached through children with f <= 0, or -1. */
+typedef struct orc_leafset orc_leafset;
+orc_leafset *orc_leafset_build(const orc_node *nodes, uint64_t n, const double root_box[6], double pad);
+void orc_leafset_free(orc_leafset *s);
+void orc_leafset_march(const orc_leafset *s, const double *origins, const double *dirs, uint64_t m,
+                       double *depth_out);
+
 #ifdef __cplusplus
 }
 #endif

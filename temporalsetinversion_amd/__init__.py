@@ -88,6 +88,24 @@ def precompile(tape, prune=True) -> dict:
             "specialized": bool(info.specialized), "fix_kept": info.fix_kept}
 
 
+def kernel_source(tape, prune=True) -> tuple[str, dict]:
+    """The generated evaluation-kernel source for a tape and pruning setting, without compiling
+    it, and the lowering's figures (n, stack rows, prune class)."""
+    if isinstance(tape, IR):
+        tape = tape.gpu_instructions
+    elif isinstance(tape, (str, bytes)):
+        tape = tape_from_ll(tape)
+    t = np.ascontiguousarray(tape, dtype=INSTR_DTYPE)
+    buf = (GpuInstruction * len(t)).from_buffer_copy(t.tobytes())
+    flags = _lib.TSI_FLAG_PRUNE_EXACT_ONLY if prune == "exact" else (0 if prune else _lib.TSI_FLAG_NO_PRUNE)
+    n = C.c_size_t(0)
+    info = _lib.TapeInfo()
+    check(lib().tsi_tape_kernel_source(buf, len(t), flags, None, 0, C.byref(n), C.byref(info)), "tsi_tape_kernel_source")
+    out = C.create_string_buffer(n.value + 1)
+    check(lib().tsi_tape_kernel_source(buf, len(t), flags, out, n.value + 1, C.byref(n), None), "tsi_tape_kernel_source")
+    return out.value.decode(), {"n": info.n, "stack": info.stack, "prune_class": info.prune_class}
+
+
 class IR:
     """Mirror of src/ir.rs `IR`: the tape of one scene."""
 

@@ -60,7 +60,7 @@ class Stats(C.Structure):
                 ("rev_evals", C.c_uint64), ("fwd_ops", C.c_uint64), ("rev_ops", C.c_uint64),
                 ("parents", C.c_uint64), ("deallocs", C.c_uint64), ("enqueued", C.c_uint64),
                 ("allocated_nodes", C.c_uint64), ("eval_launches", C.c_uint64), ("pruned_instr", C.c_uint64),
-                ("prune_launches", C.c_uint64)]
+                ("prune_launches", C.c_uint64), ("eval_kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +108,8 @@ SIGNATURES = {
     "tsi_tape_info_get": (C.c_int, [C.c_void_p, C.POINTER(TapeInfo)]),
     "tsi_last_trace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "tsi_precompile_tape": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.c_uint32, C.POINTER(TapeInfo)]),
+    "tsi_tape_kernel_source": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t,
+                                         C.POINTER(C.c_size_t), C.c_void_p]),
     "tsi_strerror": (C.c_char_p, [C.c_int]),
 }
 

@@ -856,6 +856,45 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
     return o;
 }
 
+namespace {
+
+// the unsigned decimal at s[i..], or -1
+long long read_uint(const std::string &s, size_t i) {
+    long long v = -1;
+    while (i < s.size() && s[i] >= '0' && s[i] <= '9') v = (v < 0 ? 0 : v * 10) + (s[i++] - '0');
+    return v;
+}
+
+}  // namespace
+
+std::string jit_check_bounds(const std::string &src, uint32_t stack_rows, uint32_t record_words,
+                             uint32_t mode_words) {
+    // stack rows: (stk + 64 r u)[lane]
+    for (size_t i = src.find("(stk + "); i != std::string::npos; i = src.find("(stk + ", i + 1)) {
+        const long long off = read_uint(src, i + 7);
+        if (off < 0 || off % 64 != 0 || off / 64 >= stack_rows)
+            return "stack row " + std::to_string(off / 64) + " of " + std::to_string(stack_rows) + " at offset " +
+                   std::to_string(i);
+    }
+    // liveness-record words: prec[size_t(w u) * cap] / rec[size_t(w u) * cap]
+    for (const char *pat : {"prec[size_t(", "rec[size_t("}) {
+        const size_t len = std::strlen(pat);
+        for (size_t i = src.find(pat); i != std::string::npos; i = src.find(pat, i + 1)) {
+            const long long w = read_uint(src, i + len);
+            if (w < 0) continue;   // a word index computed at run time (bounded by its loop)
+            if (uint64_t(w) >= record_words)
+                return "record word " + std::to_string(w) + " of " + std::to_string(record_words);
+        }
+    }
+    // choice columns: chbuf[64 w u]
+    for (size_t i = src.find("chbuf["); i != std::string::npos; i = src.find("chbuf[", i + 1)) {
+        const long long off = read_uint(src, i + 6);
+        if (off < 0 || off % 64 != 0 || uint64_t(off / 64) >= (mode_words ? mode_words : 1u))
+            return "choice column " + std::to_string(off / 64) + " of " + std::to_string(mode_words);
+    }
+    return std::string();
+}
+
 }  // namespace tsi
 
 // ---------------------------------------------------------------- compilation
@@ -929,13 +968,13 @@ std::map<uint64_t, std::vector<char>> g_mem;   // code objects compiled by this 
 
 }  // namespace
 
-int jit_compile(const std::string &src, std::vector<char> &code, std::string &log, bool &cache_hit, double &ms) {
+int jit_compile(const std::string &src, bool prune, std::vector<char> &code, std::string &log, bool &cache_hit,
+                double &ms) {
     const auto t0 = std::chrono::steady_clock::now();
     cache_hit = false;
     log.clear();
     std::vector<const char *> opts(std::begin(kOptions), std::end(kOptions));
-    if (src.find("typedef __attribute__((address_space(1))) float2 gfloat2;") != std::string::npos)
-        opts.insert(opts.end(), std::begin(kLeanOptions), std::end(kLeanOptions));
+    if (!prune) opts.insert(opts.end(), std::begin(kLeanOptions), std::end(kLeanOptions));
     std::string key_text = src;
     for (const char *opt : opts) key_text += std::string("\n") + opt;
     for (int i = 0; i < tsi_jit_num_headers; ++i) key_text += tsi_jit_header_texts[i];

@@ -14,8 +14,9 @@ namespace tsi {
 struct JitLayout {
     uint32_t group = 16, regions = 0, region_words = 0, minmax = 0, mode_words = 0, record_words = 0;
 };
-// (prune: the layout of pruned modules, the only ones that exchange records)
-JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune = true);
+// (prune: the layout of pruned modules, the only ones that exchange records; unpruned modules
+// use longer regions)
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune);
 
 // The domain a tape's kernels may be specialised for: every cell box inside `box` (lo xyz, hi
 // xyz), every t-window inside [t0, t1].  fix_free marks the records whose value no cell in it can
@@ -38,9 +39,19 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
                              const std::vector<RevIns> &rev, const Lowered &L, bool prune,
                              const JitDomain *domain = nullptr);
 
+// Checks every saved-operand stack row, liveness-record word and choice column the source of a
+// module addresses against what the runtime allocates for it: stack_rows rows per wavefront
+// (Lowered::stack + 1), record_words and mode_words of the pruned layout (jit_layout(.., true)).
+// Returns an empty string, or the first violation.
+std::string jit_check_bounds(const std::string &src, uint32_t stack_rows, uint32_t record_words,
+                             uint32_t mode_words);
+
 // Compiles `src` for gfx950 with hiprtc (the library's kernel headers are embedded), or finds
 // the code object in this process's cache or the on-disk cache ($TSI_JIT_CACHE, default
-// $XDG_CACHE_HOME/tsi-jit or ~/.cache/tsi-jit).  Returns TSI_OK or TSI_E_JIT (log filled).
-int jit_compile(const std::string &src, std::vector<char> &code, std::string &log, bool &cache_hit, double &ms);
+// $XDG_CACHE_HOME/tsi-jit or ~/.cache/tsi-jit).  prune: the source is a pruned module (unpruned
+// ones read the stack as global memory and are compiled without machine sinking).  Returns
+// TSI_OK or TSI_E_JIT (log filled).
+int jit_compile(const std::string &src, bool prune, std::vector<char> &code, std::string &log, bool &cache_hit,
+                double &ms);
 
 }  // namespace tsi

@@ -449,11 +449,13 @@ int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1) {
+int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1,
+                 const hipEvent_t *e_level) {
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a);
     if (e_eval0) hipEventRecord(e_eval0, s);
     for (int d = 0; d < fl.levels; ++d) {
         const bool compiled = fl.tape_eval != nullptr;
+        if (e_level) hipEventRecord(e_level[2 * d], s);
         if (a.prune && d > 0 && !compiled) {   // the interpreter's group programs
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 0);
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 1);
@@ -471,6 +473,7 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
             hipLaunchKernelGGL(k_eval<false>, dim3(fl.grid_fwd), dim3(256), 0, s, a, d, fl.time);
             hipLaunchKernelGGL(k_eval<true>, dim3(fl.grid_rev), dim3(256), 0, s, a, d, fl.time);
         }
+        if (e_level) hipEventRecord(e_level[2 * d + 1], s);
     }
     if (e_eval1) hipEventRecord(e_eval1, s);
     // levels 1..L-1 ran the cascade's steps 1..L-1 (eval_level); blocks dropped by the last

@@ -15,7 +15,10 @@ struct FrameLaunch {
     void *tape_eval;   // the tape's compiled level kernel (hipFunction_t), or null: k_eval
 };
 
-int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1);
+// e_eval0 / e_eval1 (may be null) bracket the level loop; e_level (may be null) holds an event
+// pair per level, recorded right before and after that level's evaluation launches.
+int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1,
+                 const hipEvent_t *e_level);
 int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
 int launch_enqueue_root(const KArgs &a, hipStream_t s);
 int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);

@@ -67,6 +67,7 @@ struct tsi_ctx {
     // and pinned host copies of what the host reads back
     struct FrameSlot {
         hipEvent_t begin = nullptr, end = nullptr, eval0 = nullptr, eval1 = nullptr;
+        hipEvent_t level[2 * kMaxLevels] = {};   // around each level's evaluation launches
         Counters *h = nullptr;
     };
     // a trace runs on its slot's stream over its slot's device-side snapshot of the octree, so it
@@ -191,7 +192,7 @@ int alloc_prune(tsi_ctx *c, const Lowered &L, const std::vector<FwdIns> &fwd) {
     uint64_t words = uint64_t(kRecHead + 3 * nchunks) * cap;
     uint32_t choice_len = (L.nminmax + 15) / 16 ? (L.nminmax + 15) / 16 : 1;
     if (!interp) {   // the compiled kernels' record layout (tsi_jit.h)
-        const JitLayout Y = jit_layout(fwd, L);
+        const JitLayout Y = jit_layout(fwd, L, true);
         words = uint64_t(Y.record_words) * cap;
         choice_len = Y.mode_words ? Y.mode_words : 1;
     }
@@ -236,6 +237,16 @@ void release_jit(tsi_ctx *c) {
 // kernels are compiled for it, and recompiled without that assumption if a frame leaves it.
 const JitDomain kDefaultDomain{{-5.0f, -5.0f, -5.0f, 5.0f, 5.0f, 5.0f}, -2.0f, 3.0f};
 
+// The generated module addresses no stack row, liveness-record word or choice column beyond what
+// alloc_prune and tsi_upload_tape allocate (a module that would is refused, not run).
+int check_module(const std::string &src, const std::vector<FwdIns> &fwd, const Lowered &L, std::string &log) {
+    const JitLayout Y = jit_layout(fwd, L, true);
+    const std::string bad = jit_check_bounds(src, L.stack + 1u, Y.record_words, Y.mode_words);
+    if (bad.empty()) return TSI_OK;
+    log = "generated module out of bounds: " + bad;
+    return TSI_E_JIT;
+}
+
 int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
              const std::vector<RevIns> &rev, const Lowered &L, const JitDomain *domain) {
     release_jit(c);
@@ -247,7 +258,8 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
     int rc = TSI_OK;
     try {
         src = jit_source_level(tape, n, fwd, rev, L, c->prune, domain);
-        rc = jit_compile(src, code, log, hit, ms);
+        rc = check_module(src, fwd, L, log);
+        if (rc == TSI_OK) rc = jit_compile(src, c->prune, code, log, hit, ms);
     } catch (const std::runtime_error &) {   // the generator cannot lay out this tape
         return TSI_E_JIT;
     } catch (...) {
@@ -285,6 +297,8 @@ void release(tsi_ctx *c) {
     for (auto &f : c->fslot) {
         for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1})
             if (*e) { hipEventDestroy(*e); *e = nullptr; }
+        for (hipEvent_t &e : f.level)
+            if (e) { hipEventDestroy(e); e = nullptr; }
         if (f.h) { hipHostFree(f.h); f.h = nullptr; }
     }
     for (auto &t : c->tslot) {
@@ -372,6 +386,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         for (auto &f : c->fslot) {
             for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1}) evok = evok && hipEventCreate(e) == hipSuccess;
+            for (hipEvent_t &e : f.level) evok = evok && hipEventCreate(&e) == hipSuccess;
             evok = evok && hipHostMalloc((void **)&f.h, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
         }
         for (auto &t : c->tslot) {
@@ -556,7 +571,7 @@ int tsi_set_inversion_async(tsi_ctx *c) {
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
                    (void *)c->eval_fn[0]};
     HIP_OK(hipEventRecord(f.begin, c->stream));
-    if (launch_frame(a, fl, c->stream, f.eval0, f.eval1) != 0) return TSI_E_HIP;
+    if (launch_frame(a, fl, c->stream, f.eval0, f.eval1, f.level) != 0) return TSI_E_HIP;
     HIP_OK(hipMemcpyAsync(f.h, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipEventRecord(f.end, c->stream));
     c->f_issued += 1;
@@ -578,6 +593,13 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->gpu_ms = ms;
         stats->eval_ms = ems;
+        double kms = 0.0;
+        for (int d = 0; d < c->p.localsubdiv; ++d) {
+            float x = 0.0f;
+            hipEventElapsedTime(&x, f.level[2 * d], f.level[2 * d + 1]);
+            kms += x;
+        }
+        stats->eval_kernel_ms = kms;
         stats->cell_evals = h.stats[kStatCells];
         stats->rev_evals = h.stats[kStatRevCells];
         stats->fwd_ops = h.stats[kStatFwdOps];
@@ -785,12 +807,14 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t fl
             std::vector<char> code;
             bool hit = false;
             double ms = 0.0;
-            src = jit_source_level(tape, n, fwd, rev, L, prune_wanted(flags, L), &kDefaultDomain);
+            const bool prune = prune_wanted(flags, L);
+            src = jit_source_level(tape, n, fwd, rev, L, prune, &kDefaultDomain);
             info.specialized = 1;
             info.fix_kept = n - 4u;
             const std::vector<char> ff = fix_free(tape, n, fwd, kDefaultDomain);
             for (uint32_t j = 4; j < n; ++j) info.fix_kept -= ff[j] ? 1u : 0u;
-            rc = jit_compile(src, code, log, hit, ms);
+            rc = check_module(src, fwd, L, log);
+            if (rc == TSI_OK) rc = jit_compile(src, prune, code, log, hit, ms);
             info.compile_ms = ms;
             info.cache_hit = hit;
             if (rc != TSI_OK) std::strncpy(info.log, log.c_str(), sizeof info.log - 1);
@@ -802,6 +826,41 @@ int tsi_precompile_tape(const tsi_gpu_instruction *tape, uint32_t n, uint32_t fl
     }
     if (out) *out = info;
     return rc;
+}
+
+int tsi_tape_kernel_source(const tsi_gpu_instruction *tape, uint32_t n, uint32_t flags, char *out, size_t cap,
+                           size_t *len, tsi_tape_info *info) {
+    if (!tape) return TSI_E_ARG;
+    if (n > 4096) return TSI_E_TAPE_TOO_LONG;
+    std::vector<FwdIns> fwd;
+    std::vector<RevIns> rev;
+    Lowered L;
+    std::string src;
+    try {
+        const int rc = lower(tape, n, fwd, rev, L);
+        if (rc != TSI_OK) return rc;
+        src = jit_source_level(tape, n, fwd, rev, L, prune_wanted(flags, L), &kDefaultDomain);
+    } catch (const std::runtime_error &) {
+        return TSI_E_JIT;
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->n = n;
+        info->stack = L.stack;
+        info->slots = L.slots;
+        info->prune_class = uint32_t(L.prune_class);
+        info->engine = 1;
+        info->specialized = 1;
+    }
+    if (len) *len = src.size();
+    if (out && cap) {
+        const size_t m = src.size() < cap - 1 ? src.size() : cap - 1;
+        std::memcpy(out, src.data(), m);
+        out[m] = 0;
+    }
+    return TSI_OK;
 }
 
 int tsi_last_trace_info(tsi_ctx *c, double *trace_ms, uint64_t *node_visits) {

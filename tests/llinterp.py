@@ -180,3 +180,49 @@ class Program:
                 margin = 0.0
             env[dest] = min(max(r, -BIG), BIG)
         return val(self.ret), margin
+
+    def eval_many(self, x, y, z, t) -> np.ndarray:
+        """eval() over arrays of points at once (float64; same semantics, no margin), for the
+        dense sampling of large octrees.  Supports the opcodes of the seeded scene generators."""
+        x, y, z, t = (np.asarray(a, np.float64) for a in np.broadcast_arrays(x, y, z, t))
+        env = {0: x, 1: y, 2: z, 3: t}
+
+        def val(o):
+            return env[o[1]] if o[0] == "r" else o[1]
+
+        with np.errstate(all="ignore"):
+            for dest, op, ops in self.ins:
+                a = val(ops[0])
+                b = val(ops[1]) if len(ops) > 1 else None
+                if op == "ADD":
+                    r = a + b
+                elif op == "SUB":
+                    r = a - b
+                elif op == "MUL":
+                    r = a * b
+                elif op == "DIV":
+                    r = np.divide(a, b)
+                elif op == "NEG":
+                    r = -a
+                elif op == "SQRT":
+                    r = np.sqrt(np.maximum(a, 0.0))
+                elif op == "EXP":
+                    r = np.exp(np.minimum(a, 700.0))
+                elif op == "SIN":
+                    r = np.sin(a)
+                elif op == "COS":
+                    r = np.cos(a)
+                elif op == "MIN":
+                    r = np.minimum(a, b)
+                elif op == "MAX":
+                    r = np.maximum(a, b)
+                elif op == "ABS":
+                    r = np.abs(a)
+                elif op == "FLOOR":
+                    r = np.floor(a)
+                else:
+                    raise NotImplementedError(op)
+                r = np.asarray(r, np.float64)
+                np.copyto(r, 0.0, where=np.isnan(r))
+                env[dest] = np.clip(r, -BIG, BIG, out=r if r.shape == x.shape else None)
+        return np.broadcast_to(val(self.ret), x.shape).astype(np.float64)

@@ -8,6 +8,9 @@ most 5^(2^k) = 25, 625, 390625, 1.5e11, ... so the first three squarings are fix
 record from the fourth on keeps its fix; 1/x (a division by an interval containing 0) is never
 bounded, and neither is anything computed from it.
 """
+import re
+
+import numpy as np
 import pytest
 
 import temporalsetinversion_amd as T
@@ -47,3 +50,71 @@ def test_bounded_scene_tapes_are_fix_free():
     for name in ("spheres", "capsules"):
         info = T.precompile(scenes.SCENES[name](), prune=False)
         assert info["specialized"] and info["fix_kept"] == 0, name
+
+
+# ---- addressing of the generated modules (the pruned csg module once faulted the GPU at
+# localsubdiv 10 with a variant of its stack loads that is no longer generated) ----
+
+def _layout(tape, prune_group=16):
+    """Record words and choice columns of the pruned layout, counted from the tape itself."""
+    ops = (tape["opcode_uses"] >> 16) & 0x3FFF
+    side = (tape["opcode_uses"] >> 30) & 3
+    body = slice(4, len(tape))
+    nfwd = len(tape) - 4
+    minmax = int(np.sum(((ops[body] == 17) | (ops[body] == 18)) & (side[body] <= 1)))
+    regions = (nfwd + prune_group - 1) // prune_group
+    return 2 + (regions + 31) // 32 + (minmax + 15) // 16, (minmax + 15) // 16
+
+
+def _functions(src):
+    """{name: body} of the generated __device__ functions."""
+    out = {}
+    for m in re.finditer(r"__device__ __noinline__ \w+ (\w+)\(.*?\) \{\n", src):
+        depth, i = 1, m.end()
+        while depth:
+            c = src[i]
+            depth += (c == "{") - (c == "}")
+            i += 1
+        out.setdefault(m.group(1), []).append(src[m.end():i - 1])
+    return out
+
+
+@pytest.mark.parametrize("scene", ["spheres", "capsules", "csg"])
+@pytest.mark.parametrize("prune", [True, False], ids=["pruned", "unpruned"])
+def test_generated_modules_stay_inside_their_buffers(scene, prune):
+    from temporalsetinversion_amd import scenes
+
+    tape = T.tape_from_ll(scenes.SCENES[scene]())
+    src, info = T.kernel_source(tape, prune=prune)
+    rows = info["stack"] + 1                       # rows per wavefront tsi_upload_tape allocates
+    used = [int(x) for x in re.findall(r"\(stk \+ (\d+)u\)\[lane\]", src)]
+    assert used and all(u % 64 == 0 for u in used)
+    assert max(used) // 64 < rows, (max(used) // 64, rows)
+    record_words, mode_words = _layout(tape)
+    words = [int(x) for x in re.findall(r"\brec\[size_t\((\d+)u\) \* cap\]", src)]
+    words += [int(x) for x in re.findall(r"\bprec\[size_t\((\d+)u\) \* cap\]", src)]
+    cols = [int(x) for x in re.findall(r"chbuf\[(\d+)u\]", src)]
+    if prune and info["prune_class"]:
+        assert words and cols
+        assert max(words) < record_words and min(words) >= 2
+        assert all(c % 64 == 0 for c in cols) and max(cols) // 64 < max(mode_words, 1)
+    else:
+        assert not words and not cols
+    # every saved value / code word is loaded before it is read; in unpruned modules the loads
+    # sit at function scope, so they precede every read on every path
+    for name, bodies in _functions(src).items():
+        for body in bodies:
+            if "RevOut" not in src or not name.startswith("tape_reverse"):
+                continue
+            first_load = {}
+            depth = 1
+            for line in body.split("\n"):
+                m = re.match(r"\s*(sv\d+|kr\d+) = ", line)
+                if m and m.group(1) not in first_load:
+                    first_load[m.group(1)] = depth
+                for v in re.findall(r"\b(sv\d+)\.x|\b(kr\d+) >>", line):
+                    v = v[0] or v[1]
+                    assert v in first_load, f"{name}: {v} read before any load"
+                depth += line.count("{") - line.count("}")
+            if not prune:
+                assert set(first_load.values()) == {1}, f"{name}: a load inside a region block"
