@@ -66,10 +66,12 @@ def profiled_traffic(workload: str, engine: str):
     traffic = avg = None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 5 and parts[0] == "tsi_tape_eval_level" and parts[1] == "timed":
-            avg = float(parts[4])
-        if len(parts) >= 5 and parts[0] == "tsi_tape_eval_level" and parts[1] == "hbm":
-            traffic = float(parts[4])
+        # "tsi_tape_eval_level timed <n> dispatches avg_us <us>" and
+        # "tsi_tape_eval_level hbm <n> <fetch KiB> <write KiB> <bytes per dispatch>"
+        if len(parts) >= 6 and parts[0] == "tsi_tape_eval_level" and parts[1] == "timed":
+            avg = float(parts[5])
+        if len(parts) >= 6 and parts[0] == "tsi_tape_eval_level" and parts[1] == "hbm":
+            traffic = float(parts[5])
     return traffic, os.path.relpath(path, ROOT), avg
 
 
@@ -133,6 +135,23 @@ def summarize_profile(args) -> None:
     lines.append(f"{'kernel':<40} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
     for name, (n, t) in sorted(per.items(), key=lambda x: -x[1][1]):
         lines.append(f"{name[:40]:<40} {n:>10} {t / 1e3:>10.3f} {t / n:>9.2f} {100 * t / total:>6.1f}%")
+    # per octree level (the k-th evaluation launch of a frame runs level k)
+    lvl, per_level = -1, {}
+    for r, f in zip(rows, frame_of):
+        name = _kernel(r["Kernel_Name"])
+        if name == "tsi::k_frame_begin":
+            lvl = 0
+        elif name == "tsi_tape_eval_level" and f >= skip:
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            per_level.setdefault(lvl, []).append(d)
+            lvl += 1
+        elif name == "tsi_tape_eval_level":
+            lvl += 1
+    if per_level:
+        lines += ["", "tsi_tape_eval_level per octree level, timed frames (us): level  mean  min  max"]
+        for d in sorted(per_level):
+            v = per_level[d]
+            lines.append(f"  level {d:>2} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
     lines.append("")
     lines.append("timed frames only (the figures bench.py reads):")
     for name in ("tsi_tape_eval_level", "tsi::k_trace"):
