@@ -31,7 +31,8 @@ enum {
     TSI_E_UNSUPPORTED = -9,      /* tape needs more live values than the register file holds */
     TSI_E_BUDGET = -10,          /* served form: request exceeds the HBM admission budget */
     TSI_E_JIT = -11,             /* the tape's evaluation kernels did not compile (see tsi_tape_info) */
-    TSI_E_PENDING = -12          /* frames in flight: collect a tsi_set_inversion_result first */
+    TSI_E_PENDING = -12,         /* frames in flight: collect a tsi_set_inversion_result first */
+    TSI_E_SCRATCH = -13          /* larger than a static buffer of the library (see its entry point) */
 };
 
 /* src/ir.rs:88-97 GPUInstruction, 32 bytes: opcode_uses = (op | side<<14) << 16 | uses,
@@ -200,6 +201,15 @@ int tsi_read_tape(tsi_ctx *ctx, tsi_gpu_instruction *out, uint32_t cap, uint32_t
  * rgba8_out (W*H*4 bytes), or written as a PNG file.  tsi_write_png writes any RGBA8 image. */
 int tsi_frame_rgba8(tsi_ctx *ctx, uint8_t *rgba8_out);
 int tsi_save_frame_png(tsi_ctx *ctx, const char *path);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): tsi_frame_rgba8 with
+ * its RGBA8 staging buffer in the library's static device memory (TSI_STATIC_RGBA8_MAX_PIXELS
+ * pixels, 32 MiB, one copy per GPU, held until the process ends) instead of a hipMalloc per call.
+ * Allocates nothing.  Frames of more pixels return TSI_E_SCRATCH.  All calls on one GPU share
+ * that buffer: calls that overlap on one GPU (several contexts or host threads) race. */
+#define TSI_STATIC_RGBA8_MAX_PIXELS (1u << 23)
+int tsi_frame_rgba8_noalloc(tsi_ctx *ctx, uint8_t *rgba8_out);
 int tsi_write_png(const char *path, const uint8_t *rgba8, uint32_t width, uint32_t height);
 
 /* Device time of the last tsi_trace dispatch and the octree nodes it visited (child-state

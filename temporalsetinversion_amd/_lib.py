@@ -19,7 +19,7 @@ TSI_OK = 0
 ERRORS = {
     -1: "TSI_E_OOM", -2: "TSI_E_QUEUE_FULL", -3: "TSI_E_POOL_EXHAUSTED", -4: "TSI_E_TAPE_TOO_LONG",
     -5: "TSI_E_HIP", -6: "TSI_E_ARG", -7: "TSI_E_PARSE", -8: "TSI_E_NO_TAPE", -9: "TSI_E_UNSUPPORTED",
-    -10: "TSI_E_BUDGET", -11: "TSI_E_JIT", -12: "TSI_E_PENDING",
+    -10: "TSI_E_BUDGET", -11: "TSI_E_JIT", -12: "TSI_E_PENDING", -13: "TSI_E_SCRATCH",
 }
 TSI_FLAG_NO_PRUNE = 1
 TSI_FLAG_PRUNE_EXACT_ONLY = 2
@@ -96,6 +96,7 @@ SIGNATURES = {
     "tsi_trace_totals": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),
     "tsi_frame_rgba8": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tsi_frame_rgba8_noalloc": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tsi_save_frame_png": (C.c_int, [C.c_void_p, C.c_char_p]),
     "tsi_write_png": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "tsi_frame_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

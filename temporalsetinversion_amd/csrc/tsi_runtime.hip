@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -25,10 +26,40 @@ uint32_t fwd_cost(uint32_t op, uint32_t side);
 
 using namespace tsi;
 
+namespace {
+
+// Staging buffer of tsi_frame_rgba8_noalloc in the library's static device memory: each device
+// the library runs on holds one copy (32 MiB) until the process ends.  Host code never uses the
+// array's name as a device address: tsi_create looks up each device's copy once, with that
+// device current, and every use goes through the pointer it stored.
+constexpr size_t kStaticRgba8Pixels = size_t(TSI_STATIC_RGBA8_MAX_PIXELS);
+static __device__ __align__(256) uchar4 scratch[kStaticRgba8Pixels];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uchar4 *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up on the first call for that device).
+int scratch_setup(int device, uchar4 **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return TSI_E_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TSI_E_HIP;
+        g_scratch_ptr[device] = static_cast<uchar4 *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return TSI_OK;
+}
+
+}  // namespace
+
 struct tsi_ctx {
     tsi_params p{};
     int device = 0;
     int num_cu = 256;
+    uchar4 *scratch_rgba8 = nullptr;    // this device's copy of the static staging buffer
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t nodes = 0;
@@ -351,6 +382,7 @@ const char *tsi_strerror(int code) {
     case TSI_E_BUDGET: return "request exceeds the admission budget";
     case TSI_E_JIT: return "the tape's evaluation kernels did not compile";
     case TSI_E_PENDING: return "frames in flight: collect tsi_set_inversion_result first";
+    case TSI_E_SCRATCH: return "frame larger than the static staging buffer (TSI_STATIC_RGBA8_MAX_PIXELS)";
     default: return "unknown error";
     }
 }
@@ -374,6 +406,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        if ((rc = scratch_setup(c->device, &c->scratch_rgba8))) break;
         bool sok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
         for (auto &t : c->tslot)
             sok = sok && hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking) == hipSuccess &&
@@ -752,6 +785,22 @@ int tsi_frame_rgba8(tsi_ctx *c, uint8_t *rgba8_out) {
     if (rc == TSI_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TSI_E_HIP;
     dfree(tmp);
     return rc;
+}
+
+int tsi_frame_rgba8_noalloc(tsi_ctx *c, uint8_t *rgba8_out) {
+    if (!c || !rgba8_out) return TSI_E_ARG;
+    const size_t px = size_t(c->u.resolution[0]) * c->u.resolution[1];
+    if (px > kStaticRgba8Pixels) return TSI_E_SCRATCH;
+    if (!c->scratch_rgba8) return TSI_E_HIP;
+    int rc;
+    if ((rc = harvest_all(c))) return rc;
+    const tsi_ctx::TraceSlot &ts = c->tslot[c->last_tslot];
+    if (px == 0 || c->t_issued == 0 || px > ts.px) return TSI_E_ARG;   // nothing traced at this size
+    // every pixel of the staging buffer that is copied out is written first: nothing to clear
+    if (launch_rgba8(ts.rgba, c->scratch_rgba8, uint32_t(px), c->stream) != 0) return TSI_E_HIP;
+    HIP_OK(hipMemcpyAsync(rgba8_out, c->scratch_rgba8, px * 4u, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TSI_OK;
 }
 
 int tsi_save_frame_png(tsi_ctx *c, const char *path) {
