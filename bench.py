@@ -350,6 +350,184 @@ def make_state(T, w, tape, prune, engine):
                    prune={"on": True, "off": False, "exact": "exact"}[prune], engine=engine)
 
 
+# ---------------------------------------------------------------- batched and served forms
+
+# node capacity per session by octree depth (generous: a request that still exhausts its pool is
+# re-run with 4x the nodes during calibration, before anything is timed)
+BATCH_CAPACITY = {6: 1 << 17, 7: 1 << 18, 8: 1 << 20, 9: 1 << 21, 10: 1 << 23}
+SERVED_CAPACITY = {7: 1 << 19, 8: 1 << 20, 9: 1 << 21, 10: 1 << 23}
+
+
+def _same_nodes(a, b) -> bool:
+    if len(a) != len(b):
+        return False
+    a = a[np.lexsort((a["k"], a["j"], a["i"], a["depth"]))]
+    b = b[np.lexsort((b["k"], b["j"], b["i"], b["depth"]))]
+    for f in a.dtype.names:
+        x, y = (a[f].view(np.uint32), b[f].view(np.uint32)) if a[f].dtype.kind == "f" else (a[f], b[f])
+        if not np.array_equal(x, y):
+            return False
+    return True
+
+
+def _plain_request(T, states, tapes, r, view, cap):
+    """One batched request on the plain path: the tape's context with the request's depth and
+    window, reset, one blocking set inversion and trace."""
+    st = states.get(r["tape"])
+    if st is None:
+        st = states[r["tape"]] = T.State(tapes[r["tape"]], width=1920, height=1080, localsubdiv=10,
+                                         voxel_capacity=cap, prune=False)
+    st.set_params(localsubdiv=r["localsubdiv"], swept_volume=True, swept_window=r["window"])
+    st.uniforms.resolution[0], st.uniforms.resolution[1] = r["width"], r["height"]
+    st.width, st.height = r["width"], r["height"]
+    st.reset()
+    st.update(r["time"], view)
+    res = st.set_inversion()
+    st.trace(pixels=False)
+    return st, res
+
+
+def batched_form(T, args) -> dict:
+    """The batched workload: `--batch-requests` independent requests (16 seeded sphere / capsule
+    tapes of 100-2000 records, localsubdiv 6..10, swept t-windows of width 1/30..1/2, frames
+    512x512..1920x1080), all in one submit (one persistent set-inversion launch + one trace
+    launch), against the same requests one at a time on the plain path."""
+    from temporalsetinversion_amd import scenes, tuning
+    from temporalsetinversion_amd.batch import Batch
+
+    tapes = [T.tape_from_ll(x) for x in scenes.batch_tapes()]
+    reqs = scenes.batch_requests(args.batch_requests, len(tapes))
+    view = T.diagonal_camera()
+    caps = [BATCH_CAPACITY[r["localsubdiv"]] for r in reqs]
+    b = Batch(tapes, arena_bytes=args.batch_arena_gib << 30, max_sessions=len(reqs))
+    compile_s = b.info()["compile_ms"] / 1e3
+
+    def open_all():
+        return [b.open(tape=r["tape"], localsubdiv=r["localsubdiv"], swept_volume=True, swept_window=r["window"],
+                       voxel_capacity=c, max_width=r["width"], max_height=r["height"]) for r, c in zip(reqs, caps)]
+
+    sids = open_all()
+    frames = [(s, r["time"], r["width"], r["height"], view, True) for s, r in zip(sids, reqs)]
+    # calibration (untimed): requests whose pool is too small get 4x the nodes
+    for _ in range(4):
+        res, _, _ = b.run(frames)
+        bad = [i for i, x in enumerate(res) if x["status"] != 0]
+        if not bad:
+            break
+        for i in bad:
+            caps[i] *= 4
+        for s in sids:
+            b.close_session(s)
+        sids = open_all()
+        frames = [(s, r["time"], r["width"], r["height"], view, True) for s, r in zip(sids, reqs)]
+    else:
+        raise RuntimeError("batched requests still fail after recalibration")
+    tune = tuning.autotune(b, frames, repeats=1) if not args.no_tune else None
+    # timed: whole submits of all requests
+    walls, sets, traces = [], [], []
+    for _ in range(args.batch_steps):
+        t0 = time.perf_counter()
+        res, s_ms, t_ms = b.run(frames)
+        walls.append(time.perf_counter() - t0)
+        sets.append(s_ms)
+        traces.append(t_ms)
+        assert all(x["status"] == 0 for x in res)
+    cells = sum(x["cell_evals"] for x in res)
+    rev_ops = sum(x["rev_ops"] + x["fwd_ops"] for x in res)
+    wall = float(np.median(walls))
+    # the same requests one at a time on the plain path (contexts per tape made beforehand)
+    states = {}
+    cap_of = {}
+    for r, c in zip(reqs, caps):
+        cap_of[r["tape"]] = max(cap_of.get(r["tape"], 0), c)
+    for i, r in enumerate(reqs):   # untimed pass: contexts, trace buffers at their largest
+        _plain_request(T, states, tapes, r, view, cap_of[r["tape"]])
+    t0 = time.perf_counter()
+    for r in reqs:
+        _plain_request(T, states, tapes, r, view, cap_of[r["tape"]])
+    plain_wall = time.perf_counter() - t0
+    # identical leaf sets and frames on a seeded sample of requests
+    rng = np.random.default_rng(11)
+    sample = sorted(set(rng.choice(len(reqs), min(args.batch_check, len(reqs)), replace=False).tolist()) |
+                    {int(np.argmax([r["localsubdiv"] * 10 + r["window"] for r in reqs]))})
+    same = 0
+    for i in sample:
+        r = reqs[i]
+        st, _ = _plain_request(T, states, tapes, r, view, cap_of[r["tape"]])
+        ok = _same_nodes(st.read_nodes(), b.read_nodes(sids[i]))
+        rgba, depth = st.trace()
+        brgba, bdepth = b.frame(sids[i], r["width"], r["height"])
+        ok = ok and np.array_equal(rgba.view(np.uint32), brgba.view(np.uint32)) and \
+            np.array_equal(depth.view(np.uint32), bdepth.view(np.uint32))
+        same += ok
+    for st in states.values():
+        st.close()
+    info = b.info()
+    b.close()
+    L = np.bincount([r["localsubdiv"] for r in reqs], minlength=11)[6:].tolist()
+    return {"workload": f"batched{len(reqs)}", "requests": len(reqs), "tapes": len(tapes),
+            "requests_per_localsubdiv_6_to_10": L,
+            "frames_per_s": len(reqs) / wall, "wall_ms_per_submit": wall * 1e3,
+            "set_inversion_ms": float(np.median(sets)), "trace_ms": float(np.median(traces)),
+            "cell_evals_per_submit": cells, "cell_evals_per_s": cells / wall, "ops_per_submit": rev_ops,
+            "plain_frames_per_s": len(reqs) / plain_wall, "plain_wall_ms": plain_wall * 1e3,
+            "speedup_vs_plain": plain_wall / wall,
+            "identical_to_plain": {"checked": len(sample), "identical": same},
+            "grid": {k: info[k] for k in ("grid_blocks", "waves_per_simd", "claim_units", "trace_claim_tiles",
+                                          "trace_blocks_per_cu", "stack_rows")},
+            "module_compile_s": compile_s, "module_cached": bool(info["cache_hit"]),
+            "tuning": ({"best": tune["best"], "seconds": tune["seconds"], "trials": tune["trials"]} if tune else None)}
+
+
+def served_form(T, args) -> dict:
+    """The served workload: 32 seeded capsule scenes (200-2000 records, localsubdiv 7..10,
+    Lipschitz correction, frames 512x512..1920x1080), requests arriving as a Poisson process;
+    each request is its scene's next 1/60 window.  Continuous batching (serve.Server) against the
+    plain path serving the same schedule one request at a time (serve.PlainServer)."""
+    from temporalsetinversion_amd import scenes, serve
+    from temporalsetinversion_amd.batch import Batch
+
+    tapes = [T.tape_from_ll(x) for x in scenes.served_tapes()]
+    rng = np.random.default_rng(12)
+    view = T.diagonal_camera()
+    sc = []
+    for i in range(len(tapes)):
+        L = int(rng.integers(7, 11))
+        W, H = scenes.FRAME_SIZES[int(rng.integers(0, len(scenes.FRAME_SIZES)))]
+        sc.append(serve.Scene(tape=i, localsubdiv=L, width=W, height=H, view=view,
+                              params=dict(lipschitz_corrected=True), voxel_capacity=SERVED_CAPACITY[L]))
+    reqs = serve.poisson_requests(len(sc), args.served_rate, args.served_seconds, seed=13)
+    b = Batch(tapes, arena_bytes=args.served_budget_gib << 30, max_sessions=len(sc))
+    srv = serve.Server(b, sc)
+    served = srv.run(reqs)
+    plain = serve.PlainServer(tapes, sc)
+    pl = plain.run(reqs)
+    # each scene's octree after the same windows, alone (plain) and in the mix (served)
+    same = sum(_same_nodes(plain.states[s].read_nodes(), b.read_nodes(srv.sid[s])) for s in srv.sid)
+    # solo latency: each scene's next frame alone on an idle device
+    solo = []
+    for s in sorted(srv.sid):
+        k = max([r.frame for r in reqs if r.scene == s], default=-1) + 1
+        t0 = time.perf_counter()
+        b.run([(srv.sid[s], serve.frame_time(k), sc[s].width, sc[s].height, view)])
+        solo.append(time.perf_counter() - t0)
+    info = b.info()
+    plain.close()
+    b.close()
+    solo_ms = np.asarray(solo) * 1e3
+    drop = ("latencies",)
+    return {"workload": f"served{len(sc)}", "rate_req_per_s": args.served_rate, "seconds": args.served_seconds,
+            "requests": len(reqs), "budget_gib": args.served_budget_gib,
+            "arena_used_gib": info["arena_used"] / 2**30, "sessions": len(srv.sid),
+            **{k: v for k, v in served.items() if k not in drop},
+            "solo_p50_ms": float(np.median(solo_ms)), "solo_p99_ms": float(np.percentile(solo_ms, 99)),
+            "p99_over_solo_p99": served["p99_ms"] / float(np.percentile(solo_ms, 99)),
+            "plain": {k: v for k, v in pl.items() if k not in drop},
+            "speedup_vs_plain_frames_per_s": served["frames_per_s"] / pl["frames_per_s"] if pl["frames_per_s"] else None,
+            "sessions_identical_to_plain": {"checked": len(srv.sid), "identical": int(same)},
+            "levels": [s.localsubdiv for s in sc]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +560,16 @@ def main():
     ap.add_argument("--copy-fetch", metavar="DB")
     ap.add_argument("--copy-write", metavar="DB")
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-served", action="store_true", help="skip the served form")
+    ap.add_argument("--no-tune", action="store_true", help="batched form: keep the default launch shape")
+    ap.add_argument("--batch-requests", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-check", type=int, default=8, help="batched requests compared with the plain path")
+    ap.add_argument("--batch-arena-gib", type=int, default=96)
+    ap.add_argument("--served-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--served-seconds", type=float, default=6.0)
+    ap.add_argument("--served-budget-gib", type=int, default=200, help="HBM admission budget of the served form")
     args = ap.parse_args()
     if args.summarize:
         summarize_profile(args)
@@ -561,6 +749,12 @@ def main():
     }
     if tvf is not None:
         out["temporal_vs_full"] = tvf
+    if not args.no_batched:
+        print("batched form", file=sys.stderr, flush=True)
+        out["batched"] = batched_form(T, args)
+    if not args.no_served:
+        print("served form", file=sys.stderr, flush=True)
+        out["served"] = served_form(T, args)
     if cpu is not None:
         out["cpu_baseline"] = cpu
         out["cpu_baseline_c1"] = cpu_c1

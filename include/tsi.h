@@ -255,6 +255,114 @@ int tsi_tape_from_ll(const char *text, size_t len, float time, tsi_gpu_instructi
  * as the library computes it. */
 float tsi_camera_zdist(uint32_t height);
 
+/* ---- batched and served forms (no counterpart in the reference, which renders one scene) ----
+ *
+ * A batch holds several tapes and any number of sessions.  A session is what a tsi_ctx holds for
+ * one scene -- an octree over a root box, evaluated with one of the batch's tapes and its own
+ * parameters -- kept in the batch's device arena, so that its octree stays resident between
+ * frames (the temporal reuse of compute.wgsl:3603-3614 within delta).  tsi_batch_submit runs one
+ * frame of each listed session: one persistent launch walks every session's octree levels
+ * together, its wavefronts taking work units (8 frontier entries of one session) from a device
+ * queue, then one launch traces every frame that asks for one, its wavefronts taking 8x8 pixel
+ * tiles of any frame from a counter.  Each session's frame is the one a tsi_ctx with the same
+ * tape and parameters computes: the same octree nodes, statistics and pixels, bit for bit.
+ * Sessions are admitted by their device footprint against the batch's arena (TSI_E_BUDGET).
+ * A batch is not thread-safe; submits are serialised (one in flight). */
+typedef struct tsi_batch tsi_batch;
+
+/* The batch's kernels: tapes[i] (lens[i] records) is tape i of tsi_session_desc.tape.  The
+ * module is compiled (or found in the code-object cache) at creation; arena_bytes of device
+ * memory are reserved for sessions (0: 1/2 of the device's free memory). */
+typedef struct {
+    uint64_t arena_bytes;         /* admission budget: sessions' pools, lists and frames */
+    uint32_t max_sessions;        /* sessions open at once (0 = 1024, the most) */
+    uint32_t waves_per_simd;      /* persistent grid: 1..4 wavefronts per SIMD (0 = tuned default 2) */
+    uint32_t claim_units;         /* work units (8 frontier entries each) per queue claim (0 = 1) */
+    uint32_t trace_claim_tiles;   /* 8x8 pixel tiles per trace-queue claim (0 = 4) */
+    uint32_t trace_blocks_per_cu; /* trace grid: blocks of 4 wavefronts per CU (0 = 4) */
+} tsi_batch_config;
+
+typedef struct {
+    uint32_t tape;                /* index into the batch's tapes */
+    tsi_params params;            /* localsubdiv, delta, point_sample, swept_volume, swept_window,
+                                     lipschitz_corrected, fastenabled, fast_multiplier, shading,
+                                     voxel_capacity (0 = 2^22 nodes), queue_capacity; flags,
+                                     prune_* are ignored: batches evaluate the full tape */
+    float root_box[6];            /* x.lo, y.lo, z.lo, x.hi, y.hi, z.hi (inside [-5,5]^3) */
+    uint32_t max_width, max_height;   /* largest frame it traces (0, 0: 1920 x 1080) */
+} tsi_session_desc;
+
+/* One frame of one session: uniforms.time = time; traced (width x height, view) unless width
+ * or height is 0.  flags: TSI_FRAME_RESET = start from an empty octree (tsi_reset inside the
+ * launch: a request that needs no earlier frame). */
+enum { TSI_FRAME_RESET = 1u };
+typedef struct {
+    uint32_t session;
+    float time;
+    uint32_t width, height;
+    float view[16];
+    uint32_t flags;
+} tsi_frame_req;
+
+typedef struct {
+    int32_t status;               /* TSI_OK, TSI_E_QUEUE_FULL or TSI_E_POOL_EXHAUSTED */
+    uint32_t pad;
+    tsi_stats stats;              /* as tsi_set_inversion's (gpu_ms, eval_ms: the whole batch's) */
+    uint64_t node_visits;         /* of its trace */
+} tsi_frame_result;
+
+typedef struct {
+    uint32_t tapes, grid_blocks, waves_per_simd, stack_rows;
+    uint32_t sessions_open, cache_hit;
+    uint32_t max_waves_per_simd;  /* most the kernel's registers and the stack allow */
+    uint32_t claim_units, trace_claim_tiles, trace_blocks_per_cu;
+    double compile_ms;
+    uint64_t arena_bytes, arena_used;
+    uint64_t stack_bytes;
+    char log[1024];
+} tsi_batch_info;
+
+int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *lens, uint32_t ntapes,
+                     const tsi_batch_config *cfg, tsi_batch **out);
+void tsi_batch_destroy(tsi_batch *b);
+/* Compiles a batch module into the code-object cache without a device (tsi_precompile_tape's
+ * counterpart): a later tsi_batch_create with the same tapes only loads it. */
+int tsi_batch_precompile(const tsi_gpu_instruction *const *tapes, const uint32_t *lens, uint32_t ntapes,
+                         tsi_batch_info *out);
+int tsi_batch_info_get(tsi_batch *b, tsi_batch_info *out);
+/* Launch shape between submits (the autotuner's knobs): the non-zero fields of cfg among
+ * waves_per_simd (<= tsi_batch_info.max_waves_per_simd), claim_units, trace_claim_tiles and
+ * trace_blocks_per_cu replace the batch's; arena_bytes and max_sessions are ignored. */
+int tsi_batch_tune(tsi_batch *b, const tsi_batch_config *cfg);
+/* Bytes of arena a session of this description takes (its admission footprint). */
+int tsi_batch_footprint(tsi_batch *b, const tsi_session_desc *d, uint64_t *bytes);
+/* Admits a session (TSI_E_BUDGET when its footprint does not fit in the arena's free space) and
+ * resets its octree to the root box (tsi_reset). */
+int tsi_batch_open(tsi_batch *b, const tsi_session_desc *d, uint32_t *session);
+int tsi_batch_close(tsi_batch *b, uint32_t session);
+/* tsi_reset of one session. */
+int tsi_batch_reset(tsi_batch *b, uint32_t session);
+/* One frame of each listed session (each at most once): set inversion, trace, then the root's
+ * re-enqueue (buf_main), queued on the batch's stream without waiting.  TSI_E_PENDING while the
+ * previous submit has not been collected. */
+int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n);
+/* TSI_OK when the submitted frames are done (or none is in flight), TSI_E_PENDING while they run;
+ * does not wait. */
+int tsi_batch_query(tsi_batch *b);
+/* Waits for the submitted frames; results[i] (may be NULL) for reqs[i]; set_ms / trace_ms (may
+ * be NULL): device time of the set-inversion launch and of the trace launch. */
+int tsi_batch_wait(tsi_batch *b, tsi_frame_result *results, double *set_ms, double *trace_ms);
+/* tsi_read_nodes / tsi_trace_wait's frame of one session (its last traced frame). */
+int tsi_batch_read_nodes(tsi_batch *b, uint32_t session, tsi_node *out, uint64_t cap, uint64_t *count);
+int tsi_batch_frame(tsi_batch *b, uint32_t session, float *rgba_out, float *depth_out);
+
+/* Parameters of a context between frames (no counterpart in the reference, whose PARAMETERS are
+ * shader constants): localsubdiv, delta, point_sample, swept_volume, swept_window,
+ * lipschitz_corrected, fastenabled and fast_multiplier take the new values; voxel_capacity,
+ * queue_capacity, flags, prune_* and shading must equal the context's (TSI_E_ARG otherwise).
+ * With frames in flight: TSI_E_PENDING. */
+int tsi_set_params(tsi_ctx *ctx, const tsi_params *params);
+
 /* Text of a TSI_E_* code. */
 const char *tsi_strerror(int code);
 

@@ -342,6 +342,21 @@ class State:
         check(lib().tsi_read_tape(self._ctx, out, n.value, C.byref(n)), "tsi_read_tape")
         return np.frombuffer(bytes(out), dtype=INSTR_DTYPE).copy()
 
+    def set_params(self, **kw):
+        """Runtime parameters between frames (tsi_set_params): localsubdiv, delta, point_sample,
+        swept_volume, swept_window, lipschitz_corrected, fastenabled, fast_multiplier."""
+        p = self.params
+        for k, v in kw.items():
+            if k not in ("localsubdiv", "delta", "point_sample", "swept_volume", "swept_window",
+                         "lipschitz_corrected", "fastenabled", "fast_multiplier"):
+                raise ValueError(f"{k} cannot change after tsi_create")
+            if k == "swept_window":
+                v = float(np.float32(v))
+            elif k != "delta" and k != "fast_multiplier":
+                v = int(v)
+            setattr(p, k, v)
+        check(lib().tsi_set_params(self._ctx, C.byref(p)), "tsi_set_params")
+
     def close(self):
         if self._ctx:
             lib().tsi_destroy(self._ctx)

@@ -26,6 +26,7 @@ TSI_FLAG_PRUNE_EXACT_ONLY = 2
 TSI_FLAG_INTERPRET = 4
 TSI_SHADE_NORMALS = 1
 TSI_SHADE_GRADIENT = 2
+TSI_FRAME_RESET = 1
 
 
 class TsiError(RuntimeError):
@@ -79,6 +80,33 @@ class Node(C.Structure):
                 ("has_children", C.c_uint32), ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("max_sessions", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("claim_units", C.c_uint32), ("trace_claim_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+
+
+class SessionDesc(C.Structure):
+    _fields_ = [("tape", C.c_uint32), ("params", Params), ("root_box", C.c_float * 6), ("max_width", C.c_uint32),
+                ("max_height", C.c_uint32)]
+
+
+class FrameReq(C.Structure):
+    _fields_ = [("session", C.c_uint32), ("time", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("view", C.c_float * 16), ("flags", C.c_uint32)]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("pad", C.c_uint32), ("stats", Stats), ("node_visits", C.c_uint64)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("tapes", C.c_uint32), ("grid_blocks", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("stack_rows", C.c_uint32), ("sessions_open", C.c_uint32), ("cache_hit", C.c_uint32),
+                ("max_waves_per_simd", C.c_uint32), ("claim_units", C.c_uint32), ("trace_claim_tiles", C.c_uint32),
+                ("trace_blocks_per_cu", C.c_uint32), ("compile_ms", C.c_double), ("arena_bytes", C.c_uint64), ("arena_used", C.c_uint64),
+                ("stack_bytes", C.c_uint64), ("log", C.c_char * 1024)]
+
+
 # name -> (restype, argtypes); the list every test checks against include/tsi.h
 SIGNATURES = {
     "tsi_default_params": (None, [C.POINTER(Params), C.c_int32]),
@@ -112,6 +140,23 @@ SIGNATURES = {
     "tsi_tape_kernel_source": (C.c_int, [C.POINTER(GpuInstruction), C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t,
                                          C.POINTER(C.c_size_t), C.c_void_p]),
     "tsi_strerror": (C.c_char_p, [C.c_int]),
+    "tsi_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "tsi_batch_create": (C.c_int, [C.POINTER(C.POINTER(GpuInstruction)), C.POINTER(C.c_uint32), C.c_uint32,
+                                   C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
+    "tsi_batch_destroy": (None, [C.c_void_p]),
+    "tsi_batch_precompile": (C.c_int, [C.POINTER(C.POINTER(GpuInstruction)), C.POINTER(C.c_uint32), C.c_uint32,
+                                       C.POINTER(BatchInfo)]),
+    "tsi_batch_info_get": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "tsi_batch_tune": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig)]),
+    "tsi_batch_footprint": (C.c_int, [C.c_void_p, C.POINTER(SessionDesc), C.POINTER(C.c_uint64)]),
+    "tsi_batch_open": (C.c_int, [C.c_void_p, C.POINTER(SessionDesc), C.POINTER(C.c_uint32)]),
+    "tsi_batch_close": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tsi_batch_reset": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tsi_batch_submit": (C.c_int, [C.c_void_p, C.POINTER(FrameReq), C.c_uint32]),
+    "tsi_batch_query": (C.c_int, [C.c_void_p]),
+    "tsi_batch_wait": (C.c_int, [C.c_void_p, C.POINTER(FrameResult), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tsi_batch_read_nodes": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "tsi_batch_frame": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

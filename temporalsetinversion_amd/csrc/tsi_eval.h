@@ -53,26 +53,32 @@ TSI_DI uint64_t dealloc_step(const KArgs &a, int step, uint32_t wave, uint32_t n
     return freed;
 }
 
-// One level of one frontier list: wavefront `wave` of `nwaves` strides over the list; `bwave`
-// indexes the wavefront's own buffers (saved-operand stack, choice rows), unique in the launch.
+// Per-lane tallies of the cells a wavefront processed, summed over the wavefront and added to
+// the counters once (flush_tally).
+struct LevelTally {
+    uint64_t cells = 0, rev = 0, par = 0, enq = 0, rev_ops = 0, fwd_ops = 0, pruned = 0, freed = 0;
+};
+
+// Frontier entries of one list at a level (a list that overflowed, flagged by wave_append,
+// holds list_cap valid entries).
+TSI_DI uint32_t list_entries(const KArgs &a, int level, int which) {
+    return min(a.cnt->list_count[level][which], a.list_cap);
+}
+
+// Group g (entries 8g .. 8g + 7) of one frontier list at one level: 8 parents x 8 child octants
+// on the wavefront's 64 lanes.  count = list_entries(a, level, REV); stk = the wavefront's
+// saved-operand stack (uniform base; lane l uses column l of each row).
 template <bool REV, class Engine>
-TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, uint32_t nwaves, uint32_t bwave_) {
+TSI_DI void eval_group(const KArgs &a, int level, float time, uint32_t g, uint32_t count, float2 *stk, Engine &E,
+                       Slots &Rlo, Slots &Rhi, LevelTally &tl) {
     const int which = REV ? 1 : 0;
     Counters *cnt = a.cnt;
-    // a list that overflowed (flagged by wave_append) holds list_cap valid entries
-    const uint32_t count = min(cnt->list_count[level][which], a.list_cap);
     const uint32_t *list = a.lists[level & 1][which];
     uint32_t *next_f = a.lists[(level + 1) & 1][0];
     uint32_t *next_r = a.lists[(level + 1) & 1][1];
     const uint32_t lane = lane_id();
     const uint32_t ps = lane >> 3, o = lane & 7u;
     const uint32_t lx = o & 1u, ly = (o >> 1) & 1u, lz = o >> 2;
-    // wave indices made visibly uniform: the grid-stride loop and everything inside it then
-    // compile as scalar control flow
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_);
-    const uint32_t bwave = __builtin_amdgcn_readfirstlane(bwave_);
-    // this wavefront's saved-operand stack (uniform base; lane l uses column l of each row)
-    float2 *stk = REV ? a.stack + size_t(bwave) * a.stack_len * 64 : nullptr;
     const float eps = a.epsilon;
     const bool prune = Engine::kPrune && a.prune != 0;
     // this level's parents' liveness records (written by the previous level) and the next
@@ -80,14 +86,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
     const uint32_t *prec_in = a.prune_rec[level & 1][which];
     uint32_t *prec_f = a.prune_rec[(level + 1) & 1][0];
     uint32_t *prec_r = a.prune_rec[(level + 1) & 1][1];
-    Engine E(a, bwave, lane);
-    // register file of engines that keep values in a uniformly indexed array (a kernel-local
-    // array, so that it is promoted to VGPRs; unused by straight-line engines)
-    Slots Rlo, Rhi;
-
-    uint64_t n_cells = 0, n_rev = 0, n_par = 0, n_enq = 0, rev_ops = 0, fwd_ops = 0, n_pruned = 0;
-
-    for (uint32_t g = wave; g * 8u < count; g += nwaves) {
+    {
         const uint32_t pidx = g * 8u + ps;
         const bool active = pidx < count;
         const uint32_t P = active ? list[pidx] : 0u;
@@ -119,7 +118,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
             if (o == 0) atomicOr(&cnt->frame_error, kErrPool);
         }
         if (isnew && ok && o == 0) a.st[P].w = __uint_as_float((pmeta & kFastBit) | (blk + 1u));
-        n_par += (active && o == 0) ? 1 : 0;
+        tl.par += (active && o == 0) ? 1 : 0;
 
         const uint32_t C = ok ? blk * 8u + o : 0u;
         float4 cst = make_float4(1.0f, 0.0f, 0.0f, 0.0f);   // mallocV (compute.wgsl:3437-3449)
@@ -193,8 +192,8 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
         if (need_eval) {
             const uint32_t fc = E.template forward<REV>(a, cx, cy, cz, targ, stk, field, Rlo, Rhi);
             if (guard) {
-                fwd_ops += fc;
-                n_pruned += a.nfwd - E.records();
+                tl.fwd_ops += fc;
+                tl.pruned += a.nfwd - E.records();
             }
         }
         Iv df = I_FROM(0.0f), df_dt = I_FROM(0.0f);
@@ -206,7 +205,7 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
                 Iv sens[4];
                 const uint32_t ops = E.reverse(a, stk, sens, Rlo, Rhi);
                 if (guard && td) {
-                    rev_ops += ops;
+                    tl.rev_ops += ops;
                     const Iv s0 = sens[0], s1 = sens[1], s2 = sens[2];
                     // the sign of a zero sensitivity depends on which pruned instructions ran;
                     // dx/dy/dz are stored with +0 for any zero
@@ -286,26 +285,18 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
             if (!valid) rec = nullptr;
             if (ballot(rec != nullptr) != 0) E.liveness(a, rec);
         }
-        n_cells += guard ? 1 : 0;
-        n_rev += (guard && td) ? 1 : 0;
-        n_enq += enq ? 1 : 0;
+        tl.cells += guard ? 1 : 0;
+        tl.rev += (guard && td) ? 1 : 0;
+        tl.enq += enq ? 1 : 0;
     }
-    // ---- the dealloc cascade's step for the blocks dropped by the previous level and their
-    //      descendants (forward-only kernel; the reverse kernel of the level does not touch the
-    //      list, which the previous level's kernels finished) ----
-    uint64_t n_freed = 0;
-    if (!REV && level > 0) n_freed = dealloc_step(a, level, wave, nwaves);
+}
 
-    // ---- statistics: one atomic per wavefront ----
-    n_freed = wave_sum64(n_freed);
-    n_cells = wave_sum64(n_cells);
-    n_rev = wave_sum64(n_rev);
-    n_par = wave_sum64(n_par);
-    n_enq = wave_sum64(n_enq);
-    rev_ops = wave_sum64(rev_ops);
-    fwd_ops = wave_sum64(fwd_ops);
-    n_pruned = wave_sum64(n_pruned);
-    if (lane == 0) {
+// Adds a wavefront's tallies to the counters: one atomic per statistic.
+TSI_DI void flush_tally(Counters *cnt, const LevelTally &t) {
+    const uint64_t n_cells = wave_sum64(t.cells), n_rev = wave_sum64(t.rev), n_par = wave_sum64(t.par),
+                   n_enq = wave_sum64(t.enq), rev_ops = wave_sum64(t.rev_ops), fwd_ops = wave_sum64(t.fwd_ops),
+                   n_pruned = wave_sum64(t.pruned), n_freed = wave_sum64(t.freed);
+    if (lane_id() == 0) {
         if (n_cells) atomicAdd((unsigned long long *)&cnt->stats[kStatCells], (unsigned long long)n_cells);
         if (n_rev) atomicAdd((unsigned long long *)&cnt->stats[kStatRevCells], (unsigned long long)n_rev);
         if (n_par) atomicAdd((unsigned long long *)&cnt->stats[kStatParents], (unsigned long long)n_par);
@@ -315,6 +306,83 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
         if (n_pruned) atomicAdd((unsigned long long *)&cnt->stats[kStatPruned], (unsigned long long)n_pruned);
         if (n_freed) atomicAdd((unsigned long long *)&cnt->stats[kStatDeallocs], (unsigned long long)n_freed);
     }
+}
+
+// One level of one frontier list: wavefront `wave` of `nwaves` strides over the list; `bwave`
+// indexes the wavefront's own buffers (saved-operand stack, choice rows), unique in the launch.
+template <bool REV, class Engine>
+TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, uint32_t nwaves, uint32_t bwave_) {
+    const uint32_t count = list_entries(a, level, REV ? 1 : 0);
+    // wave indices made visibly uniform: the grid-stride loop and everything inside it then
+    // compile as scalar control flow
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_);
+    const uint32_t bwave = __builtin_amdgcn_readfirstlane(bwave_);
+    float2 *stk = REV ? a.stack + size_t(bwave) * a.stack_len * 64 : nullptr;
+    Engine E(a, bwave, lane_id());
+    // register file of engines that keep values in a uniformly indexed array (a kernel-local
+    // array, so that it is promoted to VGPRs; unused by straight-line engines)
+    Slots Rlo, Rhi;
+    LevelTally tl;
+    for (uint32_t g = wave; g * 8u < count; g += nwaves) eval_group<REV, Engine>(a, level, time, g, count, stk, E, Rlo, Rhi, tl);
+    // ---- the dealloc cascade's step for the blocks dropped by the previous level and their
+    //      descendants (forward-only kernel; the reverse kernel of the level does not touch the
+    //      list, which the previous level's kernels finished) ----
+    if (!REV && level > 0) tl.freed = dealloc_step(a, level, wave, nwaves);
+    flush_tally(a.cnt, tl);
+}
+
+// ---- the frame's first and last steps (thread t of nt threads) ----
+
+// Start of a frame: clear the per-frame counters and, when buf_main has enqueued it
+// (root_pending), put the root on the level-0 list its state selects.
+TSI_DI void frame_begin(const KArgs &a, uint32_t t, uint32_t nt) {
+    Counters *c = a.cnt;
+    for (uint32_t i = t; i < uint32_t(kMaxLevels + 2); i += nt) {
+        c->list_count[i][0] = 0;
+        c->list_count[i][1] = 0;
+        c->dlist_count[i] = 0;
+    }
+    for (uint32_t i = t; i < uint32_t(kNumStats); i += nt) c->stats[i] = 0;
+    for (uint32_t i = t; i < uint32_t(kMaxLevels + 2); i += nt) c->pool_used[i] = 0;
+    if (t == 0) {
+        c->alloc_tickets = 0;
+        c->pending = 0;
+        c->frame_error = 0;
+        if (c->root_pending) {
+            c->root_pending = 0;
+            const float4 s = a.st[0];
+            const uint32_t meta = meta_of(s);
+            const float4 lo = a.blo[0], hi = a.bhi[0];
+            const float dx = (hi.x - lo.x) / 2.0f, dy = (hi.y - lo.y) / 2.0f, dz = (hi.z - lo.z) / 2.0f;
+            const float w = wmax(dx, wmax(dy, dz));
+            const bool fwd_only = !a.swept && !a.force_deriv &&
+                                  ((meta & kChildMask) == 0 || (meta & kFastBit) || w <= a.epsilon);
+            const int which = fwd_only ? 0 : 1;
+            a.lists[0][which][0] = 0u;
+            c->list_count[0][which] = 1;
+        }
+    }
+}
+
+// Free-list merge: blocks handed out from the top of the stack are gone, freed blocks go on top.
+TSI_DI void frame_end(const KArgs &a, uint32_t t, uint32_t nt) {
+    Counters *cnt = a.cnt;
+    const uint32_t T = cnt->alloc_tickets, F = cnt->free_top, P = min(cnt->pending, a.nblocks);
+    const uint32_t used = T < F ? T : F;
+    const uint32_t base = F - used;
+    for (uint32_t i = t; i < P && base + i < a.nblocks; i += nt) a.free_stack[base + i] = a.pending_free[i];
+}
+
+// After frame_end has finished everywhere: the allocator's new state (one thread).
+TSI_DI void frame_commit(const KArgs &a) {
+    Counters *cnt = a.cnt;
+    const uint32_t T = cnt->alloc_tickets, F = cnt->free_top, P = min(cnt->pending, a.nblocks);
+    const uint32_t used = T < F ? T : F;
+    uint32_t bump = cnt->bump + (T > F ? T - F : 0u);
+    cnt->bump = bump < a.nblocks - 1u ? bump : a.nblocks - 1u;
+    cnt->free_top = min(F - used + P, a.nblocks);
+    cnt->stats[kStatLive] = uint64_t(cnt->bump) - uint64_t(cnt->free_top);
+    cnt->error |= cnt->frame_error;
 }
 
 // A level kernel that is one launch for one list: wavefront index and count from the grid.

@@ -856,6 +856,44 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
     return o;
 }
 
+std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomain *domain) {
+    std::string o = "// generated by tsi_jit.cpp: the persistent batch kernel over " + std::to_string(tapes.size()) +
+                    " tapes\n";
+    o += "#include \"tsi_batch.h\"\n#include \"tsi_rules.h\"\n\nnamespace tsi {\n";
+    for (size_t i = 0; i < tapes.size(); ++i) {
+        const BatchTape &t = tapes[i];
+        const std::string f = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, false, false, domain);
+        const std::string r = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, true, false, domain);
+        o += "namespace tb" + std::to_string(i) + " {\nnamespace jf {   // forward-only list\n" + module_body(f);
+        o += "}  // namespace jf\nnamespace jr {   // reverse list\n" + module_body(r);
+        o += "}  // namespace jr\n}  // namespace tb" + std::to_string(i) + "\n";
+    }
+    // the tape of a unit is the session's, uniform over the wavefront
+    o += "struct BatchTapes {\n"
+         "    template <bool REV, class E>\n"
+         "    static TSI_DI void run(const KArgs &a, int level, float time, uint32_t g, uint32_t count, float2 *stk,\n"
+         "                           LevelTally &tl) {\n"
+         "        E eng(a, 0u, lane_id());\n"
+         "        Slots Rlo, Rhi;\n"
+         "        eval_group<REV, E>(a, level, time, g, count, stk, eng, Rlo, Rhi, tl);\n"
+         "    }\n"
+         "    template <bool REV>\n"
+         "    static TSI_DI void group(uint32_t tape, const KArgs &a, int level, float time, uint32_t g, uint32_t count,\n"
+         "                             float2 *stk, LevelTally &tl) {\n"
+         "        switch (tape) {\n";
+    for (size_t i = 0; i < tapes.size(); ++i) {
+        const std::string ns = "tb" + std::to_string(i);
+        o += "        case " + std::to_string(i) + "u:\n"
+             "            if constexpr (REV) run<true, " + ns + "::jr::TapeEngine>(a, level, time, g, count, stk, tl);\n"
+             "            else run<false, " + ns + "::jf::TapeEngine>(a, level, time, g, count, stk, tl);\n"
+             "            break;\n";
+    }
+    o += "        default: break;\n        }\n    }\n};\n}  // namespace tsi\n\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256) tsi_batch_run(tsi::BatchArgs b) {\n"
+         "    tsi::batch_run<tsi::BatchTapes>(b);\n}\n";
+    return o;
+}
+
 namespace {
 
 // the unsigned decimal at s[i..], or -1

@@ -39,6 +39,19 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
                              const std::vector<RevIns> &rev, const Lowered &L, bool prune,
                              const JitDomain *domain = nullptr);
 
+// One tape of a batch module: the tape, its lowering.
+struct BatchTape {
+    const tsi_gpu_instruction *tape;
+    uint32_t n;
+    const std::vector<FwdIns> *fwd;
+    const std::vector<RevIns> *rev;
+    const Lowered *low;
+};
+// The persistent batch kernel (tsi_batch.h) over several tapes: each tape's unpruned code of both
+// lists in a namespace of its own (tb<i>), and one kernel, tsi_batch_run(BatchArgs), that runs a
+// session's work units with the code of the session's tape.
+std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomain *domain);
+
 // Checks every saved-operand stack row, liveness-record word and choice column the source of a
 // module addresses against what the runtime allocates for it: stack_rows rows per wavefront
 // (Lowered::stack + 1), record_words and mode_words of the pruned layout (jit_layout(.., true)).

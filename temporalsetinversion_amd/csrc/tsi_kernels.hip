@@ -32,35 +32,7 @@ __global__ void __launch_bounds__(256) k_eval(KArgs a, int level, float time) {
     eval_level<REV, InterpEngine>(a, level, time);
 }
 
-__global__ void k_frame_begin(KArgs a) {
-    Counters *c = a.cnt;
-    const int t = threadIdx.x;
-    for (int i = t; i < kMaxLevels + 2; i += blockDim.x) {
-        c->list_count[i][0] = 0;
-        c->list_count[i][1] = 0;
-        c->dlist_count[i] = 0;
-    }
-    for (int i = t; i < kNumStats; i += blockDim.x) c->stats[i] = 0;
-    for (int i = t; i < kMaxLevels + 2; i += blockDim.x) c->pool_used[i] = 0;
-    if (t == 0) {
-        c->alloc_tickets = 0;
-        c->pending = 0;
-        c->frame_error = 0;
-        if (c->root_pending) {
-            c->root_pending = 0;
-            const float4 s = a.st[0];
-            const uint32_t meta = meta_of(s);
-            const float4 lo = a.blo[0], hi = a.bhi[0];
-            const float dx = (hi.x - lo.x) / 2.0f, dy = (hi.y - lo.y) / 2.0f, dz = (hi.z - lo.z) / 2.0f;
-            const float w = wmax(dx, wmax(dy, dz));
-            const bool fwd_only = !a.swept && !a.force_deriv &&
-                                  ((meta & kChildMask) == 0 || (meta & kFastBit) || w <= a.epsilon);
-            const int which = fwd_only ? 0 : 1;
-            a.lists[0][which][0] = 0u;
-            c->list_count[0][which] = 1;
-        }
-    }
-}
+__global__ void k_frame_begin(KArgs a) { frame_begin(a, threadIdx.x, blockDim.x); }
 // every parent decided.  Groups with an entry lacking a record run the full program.
 __global__ void __launch_bounds__(256) k_prune(KArgs a, int level, int which) {
     Counters *cnt = a.cnt;
@@ -165,26 +137,12 @@ __global__ void __launch_bounds__(256) k_dealloc(KArgs a, int step) {
     if (lane_id() == 0 && freed) atomicAdd((unsigned long long *)&a.cnt->stats[kStatDeallocs], (unsigned long long)freed);
 }
 
-// Free-list merge: blocks handed out from the top of the stack are gone, freed blocks go on top.
+// Free-list merge and counter commit at the end of a frame (frame_end / frame_commit, tsi_eval.h).
 __global__ void __launch_bounds__(256) k_frame_end(KArgs a) {
-    Counters *cnt = a.cnt;
-    const uint32_t T = cnt->alloc_tickets, F = cnt->free_top, P = min(cnt->pending, a.nblocks);
-    const uint32_t used = T < F ? T : F;
-    const uint32_t base = F - used;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P && base + i < a.nblocks; i += gridDim.x * blockDim.x)
-        a.free_stack[base + i] = a.pending_free[i];
+    frame_end(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-__global__ void k_frame_commit(KArgs a) {
-    Counters *cnt = a.cnt;
-    const uint32_t T = cnt->alloc_tickets, F = cnt->free_top, P = min(cnt->pending, a.nblocks);
-    const uint32_t used = T < F ? T : F;
-    uint32_t bump = cnt->bump + (T > F ? T - F : 0u);
-    cnt->bump = bump < a.nblocks - 1u ? bump : a.nblocks - 1u;
-    cnt->free_top = min(F - used + P, a.nblocks);
-    cnt->stats[kStatLive] = uint64_t(cnt->bump) - uint64_t(cnt->free_top);
-    cnt->error |= cnt->frame_error;
-}
+__global__ void k_frame_commit(KArgs a) { frame_commit(a); }
 
 __global__ void k_clear_nodes(KArgs a, uint64_t n) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
@@ -392,6 +350,33 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
     dist = d;
 }
 
+// One pixel of rt_main (compute.wgsl:4650-4688): ray_direction (4590-4594), view * vec4(d, 0)
+// (4630), trace, get_color_single's normal_viz miss colour (4510-4514) and the 0.45 gamma.
+TSI_DI void trace_pixel(const KArgs &a, const TraceArgs &t, uint32_t x, uint32_t y, uint32_t *stk, uint32_t &visits) {
+    const float xyx = (float(x) - float(t.width) / 2.0f) + 0.5f;
+    const float xyy = (float(y) - float(t.height) / 2.0f) + 0.5f;
+    V3 d = v3(xyx, -xyy, -t.zdist);
+    const float len = vlength(d);
+    d = v3(d.x / len, d.y / len, d.z / len);
+    const float *m = t.view;
+    const V3 g = v3(((m[0] * d.x + m[4] * d.y) + m[8] * d.z) + m[12] * 0.0f,
+                    ((m[1] * d.x + m[5] * d.y) + m[9] * d.z) + m[13] * 0.0f,
+                    ((m[2] * d.x + m[6] * d.y) + m[10] * d.z) + m[14] * 0.0f);
+    const V3 eye = v3(m[12], m[13], m[14]);
+    V3 c;
+    float dist;
+    trace_ray(a, stk, eye, g, t.time, c, dist, visits, t.shading, t.grad);
+    const size_t pi = size_t(y) * t.width + x;
+    float ca = 1.0f;
+    if ((t.shading & TSI_SHADE_NORMALS) && !(dist >= 0.0f)) {
+        c = v3(10.0f, 10.0f, 10.0f);
+        ca = 0.0f;
+    }
+    const float r = (0.0f + c.x) / 1.0f, gg = (0.0f + c.y) / 1.0f, b = (0.0f + c.z) / 1.0f;
+    t.rgba[pi] = make_float4(det_pow045(r), det_pow045(gg), det_pow045(b), (0.0f + ca) / 1.0f);
+    t.depth[pi] = dist;
+}
+
 __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     __shared__ uint32_t stack_lds[16 * 256];
     // a 16x16 block of pixels; each wavefront takes one 8x8 quadrant (square tiles keep the
@@ -400,34 +385,45 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     const uint32_t x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
     const uint32_t y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
     uint32_t visits = 0;
-    if (x < t.width && y < t.height) {
-        // ray_direction (compute.wgsl:4590-4594) and view * vec4(d, 0) (4630)
-        const float xyx = (float(x) - float(t.width) / 2.0f) + 0.5f;
-        const float xyy = (float(y) - float(t.height) / 2.0f) + 0.5f;
-        V3 d = v3(xyx, -xyy, -t.zdist);
-        const float len = vlength(d);
-        d = v3(d.x / len, d.y / len, d.z / len);
-        const float *m = t.view;
-        const V3 g = v3(((m[0] * d.x + m[4] * d.y) + m[8] * d.z) + m[12] * 0.0f,
-                        ((m[1] * d.x + m[5] * d.y) + m[9] * d.z) + m[13] * 0.0f,
-                        ((m[2] * d.x + m[6] * d.y) + m[10] * d.z) + m[14] * 0.0f);
-        const V3 eye = v3(m[12], m[13], m[14]);
-        V3 c;
-        float dist;
-        trace_ray(a, stack_lds + threadIdx.x, eye, g, t.time, c, dist, visits, t.shading, t.grad);
-        const size_t pi = size_t(y) * t.width + x;
-        // get_color_single: vec4(10, 10, 10, 0) on a miss with normal_viz (4510-4514)
-        float ca = 1.0f;
-        if ((t.shading & TSI_SHADE_NORMALS) && !(dist >= 0.0f)) {
-            c = v3(10.0f, 10.0f, 10.0f);
-            ca = 0.0f;
-        }
-        const float r = (0.0f + c.x) / 1.0f, gg = (0.0f + c.y) / 1.0f, b = (0.0f + c.z) / 1.0f;
-        t.rgba[pi] = make_float4(det_pow045(r), det_pow045(gg), det_pow045(b), (0.0f + ca) / 1.0f);
-        t.depth[pi] = dist;
-    }
+    if (x < t.width && y < t.height) trace_pixel(a, t, x, y, stack_lds + threadIdx.x, visits);
     uint64_t v = wave_sum64(visits);
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
+}
+
+// The frames of a batch in one persistent launch: wavefronts take 8x8 pixel tiles of any frame
+// from a counter (claim_tiles per atomic) until none is left.  Frame j's tiles are
+// [jobs[j].tile0, jobs[j + 1].tile0), row-major over ceil(width / 8) tiles per row.
+__global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles,
+                                                     uint32_t claim_tiles, uint32_t *claim) {
+    __shared__ uint32_t stack_lds[16 * 256];
+    const uint32_t lane = lane_id();
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(claim, claim_tiles);
+        base = __builtin_amdgcn_readfirstlane(bcast(base, 0));
+        if (base >= total_tiles) break;
+        const uint32_t end = min(base + claim_tiles, total_tiles);
+        for (uint32_t tile = base; tile < end; ++tile) {
+            uint32_t lo = 0, hi = njobs;   // last job with tile0 <= tile
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
+            }
+            const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+            const TraceJob &J = jobs[j];
+            KArgs a{};
+            a.st = const_cast<float4 *>(J.st);
+            a.blo = const_cast<float4 *>(J.blo);
+            a.bhi = const_cast<float4 *>(J.bhi);
+            const uint32_t local = tile - J.tile0;
+            const uint32_t x = (local % J.tiles_x) * 8u + (lane & 7u);
+            const uint32_t y = (local / J.tiles_x) * 8u + (lane >> 3);
+            uint32_t visits = 0;
+            if (x < J.t.width && y < J.t.height) trace_pixel(a, J.t, x, y, stack_lds + threadIdx.x, visits);
+            const uint64_t v = wave_sum64(visits);
+            if (lane == 0 && v) atomicAdd((unsigned long long *)J.t.visits, (unsigned long long)v);
+        }
+    }
 }
 
 // RGBA32F -> RGBA8 as the reference's frame dump converts it (src/main.rs:709-711, image 0.24
@@ -516,6 +512,22 @@ int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, 
                     uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(k_snapshot, dim3(grid), dim3(256), 0, s, a.cnt, a.st, a.blo, a.bhi, a.grad, s_st, s_blo, s_bhi,
                        s_grad, nodes);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles, uint32_t claim_tiles,
+                       uint32_t *claim, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_batch, dim3(grid), dim3(256), 0, s, jobs, njobs, total_tiles, claim_tiles, claim);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+int launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255u) / 256u < 1024u ? (n + 255u) / 256u : 1024u), dim3(256), 0, s, p,
+                       v, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "tsi.h"
 #include "tsi_kernels.h"
 
 namespace tsi {
@@ -25,5 +26,21 @@ int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
 int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, float4 *s_grad, uint64_t nodes,
                     uint32_t grid, hipStream_t s);
 int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s);
+
+// One frame of a batched trace (k_trace_batch): the session's nodes and the frame's arguments.
+struct TraceJob {
+    const float4 *st, *blo, *bhi, *grad;
+    TraceArgs t;
+    uint32_t tiles_x;   // ceil(width / 8)
+    uint32_t tile0;     // the frame's first tile in the batch
+};
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles, uint32_t claim_tiles,
+                       uint32_t *claim, uint32_t grid, hipStream_t s);
+// The canonical node records of an octree (tsi_read_nodes): every node reachable from the root
+// through nodes with children, keyed by (depth, i, j, k).  Synchronous device reads.
+int read_octree(const float4 *st_dev, const float4 *grad_dev, const Counters *cnt_dev, uint64_t nodes, tsi_node *out,
+                uint64_t cap, uint64_t *count);
+// p[0 .. n) = v (a kernel, not a memset node, so that it also replays correctly in a captured graph)
+int launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s);
 
 }  // namespace tsi

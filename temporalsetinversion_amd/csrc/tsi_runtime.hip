@@ -350,6 +350,55 @@ void release(tsi_ctx *c) {
 
 }  // namespace
 
+namespace tsi {
+
+int read_octree(const float4 *st_dev, const float4 *grad_dev, const Counters *cnt_dev, uint64_t nodes, tsi_node *out,
+                uint64_t cap, uint64_t *count) {
+    Counters h;
+    HIP_OK(hipMemcpy(&h, cnt_dev, sizeof h, hipMemcpyDeviceToHost));
+    uint64_t used = (uint64_t(h.bump) + 1u) * 8u;
+    if (used > nodes) used = nodes;
+    std::vector<float4> st, gr;
+    try {
+        st.resize(used);
+        gr.resize(used);
+    } catch (...) {
+        return TSI_E_OOM;
+    }
+    HIP_OK(hipMemcpy(st.data(), st_dev, used * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(gr.data(), grad_dev, used * sizeof(float4), hipMemcpyDeviceToHost));
+    struct Fr { uint32_t node, d, i, j, k; };
+    std::vector<Fr> stack{{0, 0, 0, 0, 0}};
+    uint64_t n = 0;
+    while (!stack.empty()) {
+        Fr f = stack.back();
+        stack.pop_back();
+        if (f.node >= used) return TSI_E_ARG;
+        const float4 s = st[f.node];
+        uint32_t meta;
+        std::memcpy(&meta, &s.w, 4);
+        const uint32_t child = meta & 0x7FFFFFFFu;
+        if (out && n < cap) {
+            tsi_node &o = out[n];
+            o.depth = f.d; o.i = f.i; o.j = f.j; o.k = f.k;
+            o.f = s.x; o.t = s.y; o.time_err = s.z;
+            o.fast = (meta & 0x80000000u) ? 1.0f : 0.0f;
+            o.has_children = child != 0;
+            o.dx = gr[f.node].x; o.dy = gr[f.node].y; o.dz = gr[f.node].z;
+        }
+        ++n;
+        if (child) {
+            for (uint32_t l = 0; l < 8; ++l)
+                stack.push_back({(child - 1u) * 8u + l, f.d + 1, 2 * f.i + (l & 1u), 2 * f.j + ((l >> 1) & 1u),
+                                 2 * f.k + (l >> 2)});
+        }
+    }
+    if (count) *count = n;
+    return TSI_OK;
+}
+
+}  // namespace tsi
+
 extern "C" {
 
 void tsi_default_params(tsi_params *p, int32_t localsubdiv) {
@@ -928,45 +977,20 @@ int tsi_tape_info_get(tsi_ctx *c, tsi_tape_info *out) {
 int tsi_read_nodes(tsi_ctx *c, tsi_node *out, uint64_t cap, uint64_t *count) {
     if (!c) return TSI_E_ARG;
     HIP_OK(hipStreamSynchronize(c->stream));
-    Counters h;
-    HIP_OK(hipMemcpy(&h, c->cnt, sizeof h, hipMemcpyDeviceToHost));
-    const uint64_t used = (uint64_t(h.bump) + 1u) * 8u;
-    std::vector<float4> st, gr;
-    try {
-        st.resize(used);
-        gr.resize(used);
-    } catch (...) {
-        return TSI_E_OOM;
-    }
-    HIP_OK(hipMemcpy(st.data(), c->st, used * sizeof(float4), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(gr.data(), c->grad, used * sizeof(float4), hipMemcpyDeviceToHost));
-    struct Fr { uint32_t node, d, i, j, k; };
-    std::vector<Fr> stack{{0, 0, 0, 0, 0}};
-    uint64_t n = 0;
-    while (!stack.empty()) {
-        Fr f = stack.back();
-        stack.pop_back();
-        if (f.node >= used) return TSI_E_ARG;
-        const float4 s = st[f.node];
-        uint32_t meta;
-        std::memcpy(&meta, &s.w, 4);
-        const uint32_t child = meta & 0x7FFFFFFFu;
-        if (out && n < cap) {
-            tsi_node &o = out[n];
-            o.depth = f.d; o.i = f.i; o.j = f.j; o.k = f.k;
-            o.f = s.x; o.t = s.y; o.time_err = s.z;
-            o.fast = (meta & 0x80000000u) ? 1.0f : 0.0f;
-            o.has_children = child != 0;
-            o.dx = gr[f.node].x; o.dy = gr[f.node].y; o.dz = gr[f.node].z;
-        }
-        ++n;
-        if (child) {
-            for (uint32_t l = 0; l < 8; ++l)
-                stack.push_back({(child - 1u) * 8u + l, f.d + 1, 2 * f.i + (l & 1u), 2 * f.j + ((l >> 1) & 1u),
-                                 2 * f.k + (l >> 2)});
-        }
-    }
-    if (count) *count = n;
+    return read_octree(c->st, c->grad, c->cnt, c->nodes, out, cap, count);
+}
+
+int tsi_set_params(tsi_ctx *c, const tsi_params *p) {
+    if (!c || !p) return TSI_E_ARG;
+    if (c->f_issued != c->f_done) return TSI_E_PENDING;
+    const tsi_params &o = c->p;
+    if (p->voxel_capacity != o.voxel_capacity || p->queue_capacity != o.queue_capacity || p->flags != o.flags ||
+        p->prune_capacity != o.prune_capacity || p->prune_live_max != o.prune_live_max || p->shading != o.shading)
+        return TSI_E_ARG;
+    if (p->localsubdiv < 1 || p->localsubdiv > kMaxLevels - 4 || !std::isfinite(p->delta) ||
+        !std::isfinite(p->swept_window) || !std::isfinite(p->fast_multiplier))
+        return TSI_E_ARG;
+    c->p = *p;
     return TSI_OK;
 }
 

@@ -197,3 +197,49 @@ SCENES = {
     "capsules": capsules,
     "csg": csg,
 }
+
+
+# ---- the batched and served workloads ----
+
+FRAME_SIZES = [(512, 512), (640, 480), (800, 600), (1024, 768), (1024, 1024), (1280, 720), (1600, 900), (1920, 1080)]
+
+
+def batch_tapes(count: int = 16, seed: int = 4) -> list:
+    """Tapes of the batched workload: `count` seeded tapes alternating the moving-spheres and the
+    capsules generators, sized for about 100-2000 tape records (a sphere takes ~17, a capsule ~37)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        if i % 2 == 0:
+            out.append(spheres(seed=1000 + i, n=int(rng.integers(6, 118))))
+        else:
+            out.append(capsules(seed=1000 + i, n=int(rng.integers(3, 55))))
+    return out
+
+
+def served_tapes(count: int = 32, seed: int = 5) -> list:
+    """Tapes of the served workload: `count` seeded capsule scenes of about 200-2000 records."""
+    rng = np.random.default_rng(seed)
+    return [capsules(seed=2000 + i, n=int(rng.integers(6, 54))) for i in range(count)]
+
+
+def batch_requests(count: int = 256, ntapes: int = 16, seed: int = 6, levels=(6, 10), sizes=None) -> list:
+    """Independent requests of the batched workload: tape ~ U{0..ntapes-1}, localsubdiv ~ U{levels},
+    a swept t-window [t1 - w, t1] with w ~ U[1/30, 1/2] and t1 ~ U[w, 1], a frame size from
+    512x512 to 1920x1080."""
+    rng = np.random.default_rng(seed)
+    sizes = sizes or FRAME_SIZES
+    out = []
+    for _ in range(count):
+        w = float(np.float32(rng.uniform(1.0 / 30.0, 0.5)))
+        t1 = float(np.float32(rng.uniform(w, 1.0)))
+        W, H = sizes[int(rng.integers(0, len(sizes)))]
+        out.append(dict(tape=int(rng.integers(0, ntapes)), localsubdiv=int(rng.integers(levels[0], levels[1] + 1)),
+                        window=w, time=t1, width=W, height=H))
+    return out
+
+
+def batch_test_sets() -> dict:
+    """Tape sets of the batch parity tests (their kernels are compiled ahead by the build)."""
+    return {"mixed4": [spheres(0), capsules(1, n=6), spheres(3, n=12, linear=True), csg(2, n=16)],
+            "pair": [capsules(1, n=6), spheres(0)], "one": [spheres(0)]}
