@@ -41,12 +41,14 @@ struct BatchSession {
     uint32_t pad[2];
 };
 
-// Device control block of a run, cleared (by a fill kernel) before the launch.
+// Device control block of a run, cleared (by a fill kernel) before the launch.  The barrier's
+// counters sit in lines of their own (64 bytes apart).
+constexpr uint32_t kBarGroups = 8;
 struct BatchCtl {
-    uint32_t bar_arrive;                  // grid barrier: arrivals since the run began
-    uint32_t bar_gen;                     // barriers completed
-    uint32_t error;                       // kErrBarrier
-    uint32_t pad;
+    uint32_t bar_group[kBarGroups][16];   // grid barrier, first level: arrivals of block group g
+    uint32_t bar_top[16];                 // second level: groups complete
+    uint32_t bar_gen[16];                 // barriers completed
+    uint32_t error[16];                   // kErrBarrier
     uint32_t claim[kMaxLevels + 4];       // work-queue heads, one per level phase
 };
 
@@ -58,36 +60,47 @@ struct BatchArgs {
     uint32_t stack_len;
     uint32_t nsess;
     int32_t phases;         // levels + 2 (the deepest session's depth, then the cascade's tail)
-    uint32_t claim_units;   // units a wavefront claims per atomic
+    uint32_t claim_units;   // units a wavefront takes per claim
     uint64_t wait_ticks;    // longest wait at a grid barrier, in s_memrealtime ticks (100 MHz)
 };
 
-// Grid-wide barrier over the resident grid (all blocks run at once: cooperative launch).  The
-// first thread of each block releases the block's writes (an agent-scope fence: the L2s of
-// the XCDs are written back), arrives, waits for the last arrival, then acquires.  The wait
-// is bounded: past wait_ticks the run is flagged and every block leaves at its next barrier.
-// Returns false when the run is to be abandoned (the same answer in every thread of a block).
+// Grid-wide barrier over the resident grid (all blocks run at once: cooperative launch), in two
+// levels so that no counter takes more than a few dozen arrivals: block b arrives on the
+// counter of its group (b mod 8), the last block of a group on the top counter, and the last
+// group bumps the generation the others wait for.  Memory order: every block releases its
+// writes before arriving (an agent-scope fence: the L2s of the XCDs are written back), the last
+// arrivals of each level acquire what the earlier ones released before passing it on, and the
+// waiters acquire the generation.  The wait is bounded: past wait_ticks the run is flagged and
+// every block leaves at its next barrier.  Returns false when the run is to be abandoned (the
+// same answer in every thread of a block).
 TSI_DI bool grid_sync(const BatchArgs &b, uint32_t &gen) {
     __shared__ uint32_t ok_sh;
     __syncthreads();
     if (threadIdx.x == 0) {
+        const uint32_t nb = gridDim.x, g = blockIdx.x % kBarGroups;
+        const uint32_t ngroups = nb < kBarGroups ? nb : kBarGroups;
+        const uint32_t gsize = nb / kBarGroups + (g < nb % kBarGroups ? 1u : 0u);
         __threadfence();
-        const uint32_t target = (gen + 1u) * gridDim.x;
-        const uint32_t arrived = atomicAdd(&b.ctl->bar_arrive, 1u) + 1u;
-        if (arrived == target) {
-            __hip_atomic_store(&b.ctl->bar_gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        bool last = false;
+        if (atomicAdd(&b.ctl->bar_group[g][0], 1u) + 1u == (gen + 1u) * gsize) {
+            __threadfence();
+            last = atomicAdd(&b.ctl->bar_top[0], 1u) + 1u == (gen + 1u) * ngroups;
+        }
+        if (last) {
+            __threadfence();
+            __hip_atomic_store(&b.ctl->bar_gen[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&b.ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < gen + 1u) {
-                __builtin_amdgcn_s_sleep(2);
+            while (__hip_atomic_load(&b.ctl->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < gen + 1u) {
+                __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - t0 > b.wait_ticks) {
-                    atomicOr(&b.ctl->error, kErrBarrier);
+                    atomicOr(&b.ctl->error[0], kErrBarrier);
                     break;
                 }
             }
         }
         __threadfence();
-        ok_sh = __hip_atomic_load(&b.ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u ? 1u : 0u;
+        ok_sh = __hip_atomic_load(&b.ctl->error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u ? 1u : 0u;
     }
     __syncthreads();
     gen += 1u;
@@ -122,7 +135,8 @@ TSI_DI uint64_t dealloc_group(const KArgs &a, int step, uint32_t g) {
 }
 
 // The persistent kernel's body.  Tapes::group<REV>(tape, a, level, time, g, count, stk, tally)
-// runs eval_group with the tape's engine.  blockDim = 256 (4 wavefronts).
+// runs eval_group with the tape's engine.  One block per CU, of 4, 8 or 12 wavefronts (one to
+// three per SIMD).
 template <class Tapes>
 TSI_DI void batch_run(const BatchArgs &b) {
     // prefix sums of the phase's units over sessions, in queue order: [0, n) reverse units of
@@ -157,8 +171,11 @@ TSI_DI void batch_run(const BatchArgs &b) {
     if (!grid_sync(b, gen)) return;
 
     const uint32_t lane = lane_id();
-    const uint32_t bwave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (tid >> 6));
+    const uint32_t wpb = blockDim.x >> 6;   // wavefronts per block
+    const uint32_t nwaves = nb * wpb;
+    const uint32_t bwave = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (tid >> 6));
     float2 *stk = b.stack + size_t(bwave) * b.stack_len * 64u;
+    const uint32_t cu = b.claim_units;
 
     for (int d = 0; d < b.phases; ++d) {
         // ---- the phase's unit table (every block builds the same one) ----
@@ -184,23 +201,21 @@ TSI_DI void batch_run(const BatchArgs &b) {
         }
         const uint32_t total = pre[m];
 
-        // ---- claim units until the phase's queue is empty ----
-        for (;;) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&b.ctl->claim[d], b.claim_units);
-            base = __builtin_amdgcn_readfirstlane(bcast(base, 0));
-            if (base >= total) break;
-            const uint32_t end = min(base + b.claim_units, total);
+        // ---- units: wavefront w first takes units [cu w, cu w + cu), then claims cu at a time
+        //      from the phase's counter, which hands out units from cu * nwaves on (a phase of
+        //      fewer units than wavefronts takes no atomic) ----
+        for (uint32_t base = bwave * cu; base < total;) {
+            const uint32_t end = min(base + cu, total);
             for (uint32_t u = base; u < end; ++u) {
-                // segment k with pre[k] <= u < pre[k + 1]
+                // segment j with pre[j] <= u < pre[j + 1]
                 uint32_t lo = 0, hi = m;
                 while (hi - lo > 1u) {
                     const uint32_t mid = (lo + hi) >> 1;
                     if (pre[mid] <= u) lo = mid; else hi = mid;
                 }
-                const uint32_t k = __builtin_amdgcn_readfirstlane(lo);
-                const uint32_t kind = k / ns, s = k - kind * ns;
-                const uint32_t g = __builtin_amdgcn_readfirstlane(u - pre[k]);
+                const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+                const uint32_t kind = j / ns, s = j - kind * ns;
+                const uint32_t g = __builtin_amdgcn_readfirstlane(u - pre[j]);
                 const BatchSession &S = b.sess[s];
                 LevelTally tl;
                 if (kind == 0u)
@@ -211,6 +226,10 @@ TSI_DI void batch_run(const BatchArgs &b) {
                     tl.freed = dealloc_group(S.a, d, g);
                 flush_tally(S.a.cnt, tl);
             }
+            if (nwaves * cu >= total) break;
+            uint32_t next = 0;
+            if (lane == 0) next = atomicAdd(&b.ctl->claim[d], cu);
+            base = __builtin_amdgcn_readfirstlane(bcast(next, 0)) + nwaves * cu;
         }
         if (!grid_sync(b, gen)) return;
     }

@@ -112,7 +112,7 @@ struct tsi_batch {
     tsi_batch_info info{};
     std::vector<std::vector<tsi_gpu_instruction>> tapes;
     std::vector<Lowered> low;
-    uint32_t grid = 0, trace_grid = 0, stack_rows = 0, max_bpc = 1;
+    uint32_t grid = 0, trace_grid = 0, stack_rows = 0, max_bpc = 1, wps = 1;
     float2 *stack = nullptr;
     BatchSession *sess_d = nullptr, *sess_h = nullptr;
     BatchCtl *ctl = nullptr;
@@ -344,16 +344,18 @@ int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *le
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         b->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        // one block per CU of 4 x waves_per_simd wavefronts: the most that fit a CU is how
+        // many 256-thread blocks would (each is one wavefront per SIMD), at most 3 (the kernel's
+        // launch bound); the stack is sized for that (tsi_batch_tune may change the block)
         int per_cu = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 256, 0) != hipSuccess || per_cu < 1) {
             rc = TSI_E_HIP;
             break;
         }
-        // one block = 4 wavefronts = one per SIMD of a CU; the stack is sized for the most
-        // blocks per CU that fit (tsi_batch_tune may change the grid up to that)
-        b->max_bpc = std::min<uint32_t>(4u, uint32_t(per_cu));
+        b->max_bpc = std::min<uint32_t>(3u, uint32_t(per_cu));
         const uint32_t bpc = std::min<uint32_t>(C.waves_per_simd, b->max_bpc);
-        b->grid = uint32_t(b->num_cu) * bpc;
+        b->wps = bpc;
+        b->grid = uint32_t(b->num_cu);
         b->trace_grid = uint32_t(b->num_cu) * C.trace_blocks_per_cu;
         if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { rc = TSI_E_HIP; break; }
         for (auto &e : b->ev)
@@ -403,12 +405,11 @@ void tsi_batch_destroy(tsi_batch *b) {
 int tsi_batch_tune(tsi_batch *b, const tsi_batch_config *cfg) {
     if (!b || !cfg) return TSI_E_ARG;
     if (b->pending) return TSI_E_PENDING;
-    if (cfg->waves_per_simd > b->max_bpc || cfg->trace_blocks_per_cu > 16u) return TSI_E_ARG;
+    if (cfg->waves_per_simd > b->max_bpc || cfg->trace_blocks_per_cu > 16u || cfg->claim_units > 64u) return TSI_E_ARG;
     tsi_batch_config &C = b->cfg;
     if (cfg->waves_per_simd) {
         C.waves_per_simd = cfg->waves_per_simd;
-        b->grid = uint32_t(b->num_cu) * cfg->waves_per_simd;
-        b->info.grid_blocks = b->grid;
+        b->wps = cfg->waves_per_simd;
         b->info.waves_per_simd = cfg->waves_per_simd;
     }
     if (cfg->claim_units) C.claim_units = cfg->claim_units;
@@ -579,7 +580,7 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     args.wait_ticks = uint64_t(100000000) * 10u;   // 10 s at the 100 MHz real-time counter
     void *kp[] = {&args};
     HIP_OK(hipEventRecord(b->ev[0], b->stream));
-    HIP_OK(hipModuleLaunchCooperativeKernel(b->fn, b->grid, 1, 1, 256, 1, 1, 0, b->stream, kp));
+    HIP_OK(hipModuleLaunchCooperativeKernel(b->fn, b->grid, 1, 1, 256u * b->wps, 1, 1, 0, b->stream, kp));
     HIP_OK(hipEventRecord(b->ev[1], b->stream));
     b->traced = njobs > 0;
     if (njobs) {
@@ -630,7 +631,7 @@ int tsi_batch_wait(tsi_batch *b, tsi_frame_result *results, double *set_ms, doub
         std::memset(&r, 0, sizeof r);
         r.status = (h.frame_error & kErrPool) ? TSI_E_POOL_EXHAUSTED
                  : (h.frame_error & kErrQueue) ? TSI_E_QUEUE_FULL : TSI_OK;
-        if (ctl.error) r.status = TSI_E_HIP;
+        if (ctl.error[0]) r.status = TSI_E_HIP;
         r.stats.gpu_ms = sms;
         r.stats.eval_ms = sms;
         r.stats.cell_evals = h.stats[kStatCells];
@@ -646,7 +647,7 @@ int tsi_batch_wait(tsi_batch *b, tsi_frame_result *results, double *set_ms, doub
         r.stats.eval_kernel_ms = sms;
         if (b->job_of[i] >= 0) r.node_visits = b->visits_h[b->job_of[i]];
     }
-    return ctl.error ? TSI_E_HIP : TSI_OK;
+    return ctl.error[0] ? TSI_E_HIP : TSI_OK;
 }
 
 int tsi_batch_read_nodes(tsi_batch *b, uint32_t session, tsi_node *out, uint64_t cap, uint64_t *count) {

@@ -16,6 +16,7 @@
 
 #include <sys/stat.h>
 #include <unistd.h>
+#include <utime.h>
 
 #include <chrono>
 #include <cmath>
@@ -889,7 +890,8 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
              "            break;\n";
     }
     o += "        default: break;\n        }\n    }\n};\n}  // namespace tsi\n\n";
-    o += "extern \"C\" __global__ void __launch_bounds__(256) tsi_batch_run(tsi::BatchArgs b) {\n"
+    // one block per CU of up to 12 wavefronts (3 per SIMD)
+    o += "extern \"C\" __global__ void __launch_bounds__(768) tsi_batch_run(tsi::BatchArgs b) {\n"
          "    tsi::batch_run<tsi::BatchTapes>(b);\n}\n";
     return o;
 }
@@ -1015,7 +1017,10 @@ int jit_compile(const std::string &src, bool prune, std::vector<char> &code, std
     if (!prune) opts.insert(opts.end(), std::begin(kLeanOptions), std::end(kLeanOptions));
     std::string key_text = src;
     for (const char *opt : opts) key_text += std::string("\n") + opt;
-    for (int i = 0; i < tsi_jit_num_headers; ++i) key_text += tsi_jit_header_texts[i];
+    // the headers the source can include: the batch kernel's header only for batch modules
+    const bool batch = src.find("#include \"tsi_batch.h\"") != std::string::npos;
+    for (int i = 0; i < tsi_jit_num_headers; ++i)
+        if (batch || std::strcmp(tsi_jit_header_names[i], "tsi_batch.h") != 0) key_text += tsi_jit_header_texts[i];
     int ver_major = 0, ver_minor = 0;
     hiprtcVersion(&ver_major, &ver_minor);
     key_text += "\nhiprtc " + std::to_string(ver_major) + "." + std::to_string(ver_minor);
@@ -1031,7 +1036,10 @@ int jit_compile(const std::string &src, bool prune, std::vector<char> &code, std
         }
     }
     const std::string dir = cache_dir();
-    if (!cache_hit && read_file(dir + "/" + name, code)) cache_hit = true;
+    if (!cache_hit && read_file(dir + "/" + name, code)) {
+        cache_hit = true;
+        ::utime((dir + "/" + name).c_str(), nullptr);   // marks the entry as in use (see the build's pruning)
+    }
     if (!cache_hit) {
         hiprtcProgram prog;
         if (hiprtcCreateProgram(&prog, src.c_str(), "tsi_tape.hip", tsi_jit_num_headers, tsi_jit_header_texts,

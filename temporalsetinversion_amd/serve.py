@@ -81,6 +81,13 @@ class Server:
         self.sid = {}            # scene -> session
         self.last_used = {}
         self.idle_close_s = idle_close_s
+        self.closed = 0          # sessions closed to admit others
+        self._waiting = {}
+        self._inflight = None
+
+    def _busy(self, s: int) -> bool:
+        """The scene has a frame in flight or waiting (its session must stay open)."""
+        return bool(self._waiting.get(s)) or any(r.scene == s for r in (self._inflight or []))
 
     def _admit(self, s: int, now: float) -> bool:
         if s in self.sid:
@@ -92,10 +99,14 @@ class Server:
         except TsiError as e:
             if e.code != -10:     # TSI_E_BUDGET: wait for a session to close
                 raise
-            # close sessions idle for long enough, then retry once
+            # close sessions idle for long enough (not while a submit is in flight: closing waits
+            # for the device), then retry once
+            if self._inflight:
+                return False
             for o, t in sorted(self.last_used.items(), key=lambda x: x[1]):
-                if o in self.sid and now - t >= self.idle_close_s:
+                if o in self.sid and now - t >= self.idle_close_s and not self._busy(o):
                     self.b.close_session(self.sid.pop(o))
+                    self.closed += 1
             if s in self.sid:
                 return True
             try:
@@ -110,7 +121,7 @@ class Server:
 
     def run(self, requests: list, on_step=None) -> dict:
         reqs = sorted(requests, key=lambda r: r.arrival)
-        waiting = collections.defaultdict(collections.deque)   # scene -> requests
+        waiting = self._waiting = collections.defaultdict(collections.deque)   # scene -> requests
         backlog = collections.deque()                           # not admitted yet
         done_at = [None] * len(reqs)
         index = {id(r): i for i, r in enumerate(reqs)}
@@ -139,7 +150,7 @@ class Server:
                     cells += x["cell_evals"]
                 set_ms += s_ms
                 trace_ms += t_ms
-                inflight = None
+                inflight = self._inflight = None
                 if on_step:
                     on_step(res)
             if inflight is None:
@@ -151,7 +162,7 @@ class Server:
                         self.last_used[r.scene] = now
                         frames.append((self.sid[r.scene], frame_time(r.frame), sc.width, sc.height, sc.view))
                     self.b.submit(frames)
-                    inflight = batch
+                    inflight = self._inflight = batch
                     steps += 1
                     batch_sizes.append(len(batch))
                 elif nxt >= len(reqs) and not backlog:
@@ -163,6 +174,7 @@ class Server:
         lat = [d - r.arrival for d, r in zip(done_at, reqs)]
         span = max(done_at) - reqs[0].arrival if reqs else 0.0
         return dict(_percentiles(lat), requests=len(reqs), steps=steps, frames_per_s=len(reqs) / span if span else 0.0,
+                    sessions_closed=self.closed,
                     mean_batch=float(np.mean(batch_sizes)) if batch_sizes else 0.0,
                     max_batch=int(max(batch_sizes)) if batch_sizes else 0, set_inversion_ms=set_ms,
                     trace_ms=trace_ms, cell_evals=cells, failed=failed, span_s=span, latencies=lat)
