@@ -355,7 +355,7 @@ def make_state(T, w, tape, prune, engine):
 # node capacity per session by octree depth (generous: a request that still exhausts its pool is
 # re-run with 4x the nodes during calibration, before anything is timed)
 BATCH_CAPACITY = {6: 1 << 17, 7: 1 << 18, 8: 1 << 20, 9: 1 << 21, 10: 1 << 23}
-SERVED_CAPACITY = {7: 1 << 19, 8: 1 << 20, 9: 1 << 21, 10: 1 << 23}
+SERVED_CAPACITY = {7: 1 << 20, 8: 1 << 21, 9: 1 << 22, 10: 1 << 24}
 
 
 def _same_nodes(a, b) -> bool:
@@ -473,8 +473,7 @@ def batched_form(T, args) -> dict:
             "plain_frames_per_s": len(reqs) / plain_wall, "plain_wall_ms": plain_wall * 1e3,
             "speedup_vs_plain": plain_wall / wall,
             "identical_to_plain": {"checked": len(sample), "identical": same},
-            "grid": {k: info[k] for k in ("grid_blocks", "waves_per_simd", "claim_units", "trace_claim_tiles",
-                                          "trace_blocks_per_cu", "stack_rows")},
+            "grid": {k: info[k] for k in ("grid_blocks", "waves_per_simd", "claim_units", "stack_rows")},
             "module_compile_s": compile_s, "module_cached": bool(info["cache_hit"]),
             "tuning": ({"best": tune["best"], "seconds": tune["seconds"], "trials": tune["trials"]} if tune else None)}
 
@@ -498,10 +497,28 @@ def served_form(T, args) -> dict:
                               params=dict(lipschitz_corrected=True), voxel_capacity=SERVED_CAPACITY[L]))
     reqs = serve.poisson_requests(len(sc), args.served_rate, args.served_seconds, seed=13)
     b = Batch(tapes, arena_bytes=args.served_budget_gib << 30, max_sessions=len(sc))
+    b.tune(waves_per_simd=b.info()["max_waves_per_simd"])
     srv = serve.Server(b, sc)
     served = srv.run(reqs)
     plain = serve.PlainServer(tapes, sc)
     pl = plain.run(reqs)
+    # saturation: an offered load above what either path sustains (requests continue each scene's
+    # frame sequence); frames/s is then each path's capacity
+    sat = None
+    if args.served_saturation_rate > 0:
+        more = serve.poisson_requests(len(sc), args.served_saturation_rate, args.served_saturation_seconds, seed=14)
+        done = [max([r.frame for r in reqs if r.scene == s], default=-1) + 1 for s in range(len(sc))]
+        for r in more:
+            r.frame += done[r.scene]
+        s_sat = srv.run(more)
+        p_sat = plain.run(more)
+        reqs = reqs + more
+        sat = {"rate_req_per_s": args.served_saturation_rate, "seconds": args.served_saturation_seconds,
+               "requests": len(more), "frames_per_s": s_sat["frames_per_s"], "p50_ms": s_sat["p50_ms"],
+               "p99_ms": s_sat["p99_ms"], "mean_batch": s_sat["mean_batch"], "failed": s_sat["failed"],
+               "plain_frames_per_s": p_sat["frames_per_s"], "plain_p50_ms": p_sat["p50_ms"],
+               "plain_p99_ms": p_sat["p99_ms"], "plain_failed": p_sat["failed"],
+               "speedup_vs_plain_frames_per_s": s_sat["frames_per_s"] / p_sat["frames_per_s"]}
     # each scene's octree after the same windows, alone (plain) and in the mix (served)
     same = sum(_same_nodes(plain.states[s].read_nodes(), b.read_nodes(srv.sid[s])) for s in srv.sid)
     # solo latency: each scene's next frame alone on an idle device
@@ -522,9 +539,11 @@ def served_form(T, args) -> dict:
             **{k: v for k, v in served.items() if k not in drop},
             "solo_p50_ms": float(np.median(solo_ms)), "solo_p99_ms": float(np.percentile(solo_ms, 99)),
             "p99_over_solo_p99": served["p99_ms"] / float(np.percentile(solo_ms, 99)),
+            "p99_over_solo_p50": served["p99_ms"] / float(np.median(solo_ms)),
             "plain": {k: v for k, v in pl.items() if k not in drop},
             "speedup_vs_plain_frames_per_s": served["frames_per_s"] / pl["frames_per_s"] if pl["frames_per_s"] else None,
             "sessions_identical_to_plain": {"checked": len(srv.sid), "identical": int(same)},
+            "saturation": sat,
             "levels": [s.localsubdiv for s in sc]}
 
 
@@ -567,8 +586,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-check", type=int, default=8, help="batched requests compared with the plain path")
     ap.add_argument("--batch-arena-gib", type=int, default=96)
-    ap.add_argument("--served-rate", type=float, default=200.0, help="requests per second")
-    ap.add_argument("--served-seconds", type=float, default=6.0)
+    ap.add_argument("--served-rate", type=float, default=400.0, help="requests per second")
+    ap.add_argument("--served-seconds", type=float, default=5.0)
+    ap.add_argument("--served-saturation-rate", type=float, default=3000.0,
+                    help="offered load above either path's capacity (0: skip)")
+    ap.add_argument("--served-saturation-seconds", type=float, default=1.5)
     ap.add_argument("--served-budget-gib", type=int, default=200, help="HBM admission budget of the served form")
     args = ap.parse_args()
     if args.summarize:

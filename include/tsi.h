@@ -225,7 +225,7 @@ typedef struct {
     uint32_t engine;              /* 0 interpreter, 1 compiled straight-line kernels */
     uint32_t cache_hit;           /* compiled kernels came from the code-object cache */
     uint32_t specialized;         /* the kernels assume every cell box inside [-5,5]^3 and every
-                                     t-window inside [-2, 3] (rebuilt without it otherwise) */
+                                     t-window inside [-2, 64] (rebuilt without it otherwise) */
     uint32_t fix_kept;            /* records whose fix is evaluated (the rest provably never clamp) */
     double compile_ms;            /* hiprtc compile (or cache lookup) wall time */
     char log[1024];               /* compiler log of a failed compile (truncated) */
@@ -263,8 +263,8 @@ float tsi_camera_zdist(uint32_t height);
  * frames (the temporal reuse of compute.wgsl:3603-3614 within delta).  tsi_batch_submit runs one
  * frame of each listed session: one persistent launch walks every session's octree levels
  * together, its wavefronts taking work units (8 frontier entries of one session) from a device
- * queue, then one launch traces every frame that asks for one, its wavefronts taking 8x8 pixel
- * tiles of any frame from a counter.  Each session's frame is the one a tsi_ctx with the same
+ * queue, then one launch traces every frame that asks for one (16x16-pixel blocks of all the
+ * frames).  Each session's frame is the one a tsi_ctx with the same
  * tape and parameters computes: the same octree nodes, statistics and pixels, bit for bit.
  * Sessions are admitted by their device footprint against the batch's arena (TSI_E_BUDGET).
  * A batch is not thread-safe; submits are serialised (one in flight). */
@@ -276,10 +276,8 @@ typedef struct tsi_batch tsi_batch;
 typedef struct {
     uint64_t arena_bytes;         /* admission budget: sessions' pools, lists and frames */
     uint32_t max_sessions;        /* sessions open at once (0 = 1024, the most) */
-    uint32_t waves_per_simd;      /* persistent grid: 1..4 wavefronts per SIMD (0 = tuned default 2) */
+    uint32_t waves_per_simd;      /* persistent grid: 1..3 wavefronts per SIMD (0 = 2) */
     uint32_t claim_units;         /* work units (8 frontier entries each) per queue claim (0 = 1) */
-    uint32_t trace_claim_tiles;   /* 8x8 pixel tiles per trace-queue claim (0 = 4) */
-    uint32_t trace_blocks_per_cu; /* trace grid: blocks of 4 wavefronts per CU (0 = 4) */
 } tsi_batch_config;
 
 typedef struct {
@@ -315,7 +313,7 @@ typedef struct {
     uint32_t tapes, grid_blocks, waves_per_simd, stack_rows;
     uint32_t sessions_open, cache_hit;
     uint32_t max_waves_per_simd;  /* most the kernel's registers and the stack allow */
-    uint32_t claim_units, trace_claim_tiles, trace_blocks_per_cu;
+    uint32_t claim_units;
     double compile_ms;
     uint64_t arena_bytes, arena_used;
     uint64_t stack_bytes;
@@ -331,8 +329,8 @@ int tsi_batch_precompile(const tsi_gpu_instruction *const *tapes, const uint32_t
                          tsi_batch_info *out);
 int tsi_batch_info_get(tsi_batch *b, tsi_batch_info *out);
 /* Launch shape between submits (the autotuner's knobs): the non-zero fields of cfg among
- * waves_per_simd (<= tsi_batch_info.max_waves_per_simd), claim_units, trace_claim_tiles and
- * trace_blocks_per_cu replace the batch's; arena_bytes and max_sessions are ignored. */
+ * waves_per_simd (<= tsi_batch_info.max_waves_per_simd) and claim_units replace the batch's;
+ * arena_bytes and max_sessions are ignored. */
 int tsi_batch_tune(tsi_batch *b, const tsi_batch_config *cfg);
 /* Bytes of arena a session of this description takes (its admission footprint). */
 int tsi_batch_footprint(tsi_batch *b, const tsi_session_desc *d, uint64_t *bytes);

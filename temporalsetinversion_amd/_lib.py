@@ -82,7 +82,7 @@ class Node(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("max_sessions", C.c_uint32), ("waves_per_simd", C.c_uint32),
-                ("claim_units", C.c_uint32), ("trace_claim_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+                ("claim_units", C.c_uint32)]
 
 
 class SessionDesc(C.Structure):
@@ -102,8 +102,7 @@ class FrameResult(C.Structure):
 class BatchInfo(C.Structure):
     _fields_ = [("tapes", C.c_uint32), ("grid_blocks", C.c_uint32), ("waves_per_simd", C.c_uint32),
                 ("stack_rows", C.c_uint32), ("sessions_open", C.c_uint32), ("cache_hit", C.c_uint32),
-                ("max_waves_per_simd", C.c_uint32), ("claim_units", C.c_uint32), ("trace_claim_tiles", C.c_uint32),
-                ("trace_blocks_per_cu", C.c_uint32), ("compile_ms", C.c_double), ("arena_bytes", C.c_uint64), ("arena_used", C.c_uint64),
+                ("max_waves_per_simd", C.c_uint32), ("claim_units", C.c_uint32), ("compile_ms", C.c_double), ("arena_bytes", C.c_uint64), ("arena_used", C.c_uint64),
                 ("stack_bytes", C.c_uint64), ("log", C.c_char * 1024)]
 
 

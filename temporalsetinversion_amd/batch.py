@@ -3,7 +3,7 @@
 No counterpart in the reference, whose host (src/main.rs) renders one scene per frame.  A
 `Batch` holds several tapes and sessions; a session is one scene's octree (what `State` holds),
 resident in the batch's device arena.  `Batch.run(frames)` computes one frame of each listed
-session in one persistent set-inversion launch and one tile-queue trace launch; each session's
+session in one persistent set-inversion launch and one trace launch; each session's
 nodes and pixels are those `State` computes for the same tape, parameters and times.
 """
 from __future__ import annotations
@@ -52,10 +52,9 @@ class Batch:
     """Sessions of several tapes in one device arena, one launch per list of frames."""
 
     def __init__(self, tapes, arena_bytes: int = 0, max_sessions: int = 0, waves_per_simd: int = 0,
-                 claim_units: int = 0, trace_claim_tiles: int = 0, trace_blocks_per_cu: int = 0):
+                 claim_units: int = 0):
         self._bufs, ptrs, lens = tape_pointers(tapes)
-        cfg = _lib.BatchConfig(arena_bytes, max_sessions, waves_per_simd, claim_units, trace_claim_tiles,
-                               trace_blocks_per_cu)
+        cfg = _lib.BatchConfig(arena_bytes, max_sessions, waves_per_simd, claim_units)
         self._b = C.c_void_p()
         check(lib().tsi_batch_create(ptrs, lens, len(self._bufs), C.byref(cfg), C.byref(self._b)), "tsi_batch_create")
         self.sessions = {}   # id -> (max_width, max_height)
@@ -166,10 +165,9 @@ class Batch:
         check(lib().tsi_batch_frame(self._b, sid, rgba.ctypes.data, depth.ctypes.data), "tsi_batch_frame")
         return rgba, depth
 
-    def tune(self, waves_per_simd: int = 0, claim_units: int = 0, trace_claim_tiles: int = 0,
-             trace_blocks_per_cu: int = 0):
+    def tune(self, waves_per_simd: int = 0, claim_units: int = 0):
         """Launch shape for the next submits (0 keeps a knob as it is)."""
-        cfg = _lib.BatchConfig(0, 0, waves_per_simd, claim_units, trace_claim_tiles, trace_blocks_per_cu)
+        cfg = _lib.BatchConfig(0, 0, waves_per_simd, claim_units)
         check(lib().tsi_batch_tune(self._b, C.byref(cfg)), "tsi_batch_tune")
 
     def info(self) -> dict:

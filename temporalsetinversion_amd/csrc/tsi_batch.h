@@ -43,10 +43,8 @@ struct BatchSession {
 
 // Device control block of a run, cleared (by a fill kernel) before the launch.  The barrier's
 // counters sit in lines of their own (64 bytes apart).
-constexpr uint32_t kBarGroups = 8;
 struct BatchCtl {
-    uint32_t bar_group[kBarGroups][16];   // grid barrier, first level: arrivals of block group g
-    uint32_t bar_top[16];                 // second level: groups complete
+    uint32_t bar_count[16];               // grid barrier: arrivals since the run began
     uint32_t bar_gen[16];                 // barriers completed
     uint32_t error[16];                   // kErrBarrier
     uint32_t claim[kMaxLevels + 4];       // work-queue heads, one per level phase
@@ -64,42 +62,34 @@ struct BatchArgs {
     uint64_t wait_ticks;    // longest wait at a grid barrier, in s_memrealtime ticks (100 MHz)
 };
 
-// Grid-wide barrier over the resident grid (all blocks run at once: cooperative launch), in two
-// levels so that no counter takes more than a few dozen arrivals: block b arrives on the
-// counter of its group (b mod 8), the last block of a group on the top counter, and the last
-// group bumps the generation the others wait for.  Memory order: every block releases its
-// writes before arriving (an agent-scope fence: the L2s of the XCDs are written back), the last
-// arrivals of each level acquire what the earlier ones released before passing it on, and the
-// waiters acquire the generation.  The wait is bounded: past wait_ticks the run is flagged and
-// every block leaves at its next barrier.  Returns false when the run is to be abandoned (the
-// same answer in every thread of a block).
+// Grid-wide barrier over the resident grid (all blocks run at once: cooperative launch): the
+// first thread of each block arrives on one counter with a release (its block's writes, which
+// __syncthreads has completed, are written back from the XCD's L2), the last arrival acquires
+// them all and publishes the next generation with a release, and the others poll the generation
+// and acquire.  (A two-level tree of counters measured slower at one block per CU: the fences,
+// not the atomics, set a barrier's cost.)  The wait is bounded: past wait_ticks the run is
+// flagged and every block leaves at its next barrier.  Returns false when the run is to be
+// abandoned (the same answer in every thread of a block).
 TSI_DI bool grid_sync(const BatchArgs &b, uint32_t &gen) {
     __shared__ uint32_t ok_sh;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t nb = gridDim.x, g = blockIdx.x % kBarGroups;
-        const uint32_t ngroups = nb < kBarGroups ? nb : kBarGroups;
-        const uint32_t gsize = nb / kBarGroups + (g < nb % kBarGroups ? 1u : 0u);
-        __threadfence();
-        bool last = false;
-        if (atomicAdd(&b.ctl->bar_group[g][0], 1u) + 1u == (gen + 1u) * gsize) {
-            __threadfence();
-            last = atomicAdd(&b.ctl->bar_top[0], 1u) + 1u == (gen + 1u) * ngroups;
-        }
-        if (last) {
-            __threadfence();
+        const uint32_t target = (gen + 1u) * gridDim.x;
+        const uint32_t old = __hip_atomic_fetch_add(&b.ctl->bar_count[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == target) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __hip_atomic_store(&b.ctl->bar_gen[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&b.ctl->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < gen + 1u) {
+            while (__hip_atomic_load(&b.ctl->bar_gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen + 1u) {
                 __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - t0 > b.wait_ticks) {
                     atomicOr(&b.ctl->error[0], kErrBarrier);
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __threadfence();
         ok_sh = __hip_atomic_load(&b.ctl->error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u ? 1u : 0u;
     }
     __syncthreads();
@@ -118,7 +108,8 @@ TSI_DI void batch_units(const BatchSession &S, int d, uint32_t &nr, uint32_t &nf
 }
 
 // One dealloc unit: group g of the cascade's list `step` (dealloc_step's loop body).
-TSI_DI uint64_t dealloc_group(const KArgs &a, int step, uint32_t g) {
+template <class KA>
+TSI_DI uint64_t dealloc_group(const KA &a, int step, uint32_t g) {
     Counters *cnt = a.cnt;
     const uint32_t count = min(cnt->dlist_count[step], a.dlist_cap);
     const uint32_t *in = a.dlists[step & 1];
@@ -176,6 +167,9 @@ TSI_DI void batch_run(const BatchArgs &b) {
     const uint32_t bwave = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (tid >> 6));
     float2 *stk = b.stack + size_t(bwave) * b.stack_len * 64u;
     const uint32_t cu = b.claim_units;
+    // the first round's units go to the blocks in turn (wavefront w of block i takes slot
+    // w * nb + i), so that a phase of a few units runs on as many CUs
+    const uint32_t slot = __builtin_amdgcn_readfirstlane((tid >> 6) * nb + blockIdx.x);
 
     for (int d = 0; d < b.phases; ++d) {
         // ---- the phase's unit table (every block builds the same one) ----
@@ -201,11 +195,11 @@ TSI_DI void batch_run(const BatchArgs &b) {
         }
         const uint32_t total = pre[m];
 
-        // ---- units: wavefront w first takes units [cu w, cu w + cu), then claims cu at a time
-        //      from the phase's counter, which hands out units from cu * nwaves on (a phase of
-        //      fewer units than wavefronts takes no atomic) ----
-        for (uint32_t base = bwave * cu; base < total;) {
-            const uint32_t end = min(base + cu, total);
+        // ---- units: in the first round wavefront slot i takes unit i; then wavefronts claim cu
+        //      units at a time from the phase's counter, which hands out units from nwaves on
+        //      (a phase of no more units than wavefronts takes no atomic) ----
+        for (uint32_t base = slot, take = 1u; base < total; take = cu) {
+            const uint32_t end = min(base + take, total);
             for (uint32_t u = base; u < end; ++u) {
                 // segment j with pre[j] <= u < pre[j + 1]
                 uint32_t lo = 0, hi = m;
@@ -216,20 +210,25 @@ TSI_DI void batch_run(const BatchArgs &b) {
                 const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
                 const uint32_t kind = j / ns, s = j - kind * ns;
                 const uint32_t g = __builtin_amdgcn_readfirstlane(u - pre[j]);
-                const BatchSession &S = b.sess[s];
+                // the session table is written before the launch and only read here: a unit's
+                // kernel arguments come from scalar loads, as a launch's own arguments do
+                const auto &S = as_const(b.sess[s]);
+                const auto &a = S.a;
+                const uint32_t tape = S.tape;
+                const float time = S.time;
                 LevelTally tl;
                 if (kind == 0u)
-                    Tapes::template group<true>(S.tape, S.a, d, S.time, g, list_entries(S.a, d, 1), stk, tl);
+                    Tapes::template group<true>(tape, a, d, time, g, list_entries(a, d, 1), stk, tl);
                 else if (kind == 1u)
-                    Tapes::template group<false>(S.tape, S.a, d, S.time, g, list_entries(S.a, d, 0), stk, tl);
+                    Tapes::template group<false>(tape, a, d, time, g, list_entries(a, d, 0), stk, tl);
                 else
-                    tl.freed = dealloc_group(S.a, d, g);
-                flush_tally(S.a.cnt, tl);
+                    tl.freed = dealloc_group(a, d, g);
+                flush_tally(a.cnt, tl);
             }
-            if (nwaves * cu >= total) break;
+            if (nwaves >= total) break;
             uint32_t next = 0;
             if (lane == 0) next = atomicAdd(&b.ctl->claim[d], cu);
-            base = __builtin_amdgcn_readfirstlane(bcast(next, 0)) + nwaves * cu;
+            base = __builtin_amdgcn_readfirstlane(bcast(next, 0)) + nwaves;
         }
         if (!grid_sync(b, gen)) return;
     }

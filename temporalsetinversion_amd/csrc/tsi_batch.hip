@@ -112,11 +112,10 @@ struct tsi_batch {
     tsi_batch_info info{};
     std::vector<std::vector<tsi_gpu_instruction>> tapes;
     std::vector<Lowered> low;
-    uint32_t grid = 0, trace_grid = 0, stack_rows = 0, max_bpc = 1, wps = 1;
+    uint32_t grid = 0, stack_rows = 0, max_bpc = 1, wps = 1;
     float2 *stack = nullptr;
     BatchSession *sess_d = nullptr, *sess_h = nullptr;
     BatchCtl *ctl = nullptr;
-    uint32_t *trace_claim = nullptr;
     Counters *out_d = nullptr, *out_h = nullptr;
     TraceJob *jobs_d = nullptr, *jobs_h = nullptr;
     unsigned long long *visits_d = nullptr, *visits_h = nullptr;
@@ -172,7 +171,7 @@ void release(tsi_batch *b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->mod) hipModuleUnload(b->mod);
     b->mod = nullptr;
-    dfree(b->stack); dfree(b->sess_d); dfree(b->ctl); dfree(b->trace_claim); dfree(b->out_d); dfree(b->jobs_d);
+    dfree(b->stack); dfree(b->sess_d); dfree(b->ctl); dfree(b->out_d); dfree(b->jobs_d);
     dfree(b->visits_d); dfree(b->arena);
     hfree(b->sess_h); hfree(b->out_h); hfree(b->jobs_h); hfree(b->visits_h);
     for (auto &e : b->ev)
@@ -327,8 +326,6 @@ int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *le
     if (C.waves_per_simd == 0) C.waves_per_simd = 2;
     if (C.waves_per_simd > 4) C.waves_per_simd = 4;
     if (C.claim_units == 0) C.claim_units = 1;
-    if (C.trace_claim_tiles == 0) C.trace_claim_tiles = 4;
-    if (C.trace_blocks_per_cu == 0) C.trace_blocks_per_cu = 4;
     int rc = TSI_OK;
     do {
         // ---- tapes: lowering, then the module of all of them ----
@@ -356,14 +353,13 @@ int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *le
         const uint32_t bpc = std::min<uint32_t>(C.waves_per_simd, b->max_bpc);
         b->wps = bpc;
         b->grid = uint32_t(b->num_cu);
-        b->trace_grid = uint32_t(b->num_cu) * C.trace_blocks_per_cu;
         if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { rc = TSI_E_HIP; break; }
         for (auto &e : b->ev)
             if (hipEventCreate(&e) != hipSuccess) { rc = TSI_E_HIP; break; }
         if (rc) break;
         const size_t stack_elems = size_t(b->num_cu) * b->max_bpc * 4u * 64u * b->stack_rows;
         if ((rc = dmalloc(&b->stack, stack_elems)) || (rc = dmalloc(&b->sess_d, C.max_sessions)) ||
-            (rc = dmalloc(&b->ctl, 1)) || (rc = dmalloc(&b->trace_claim, 1)) ||
+            (rc = dmalloc(&b->ctl, 1)) ||
             (rc = dmalloc(&b->out_d, C.max_sessions)) || (rc = dmalloc(&b->jobs_d, C.max_sessions)) ||
             (rc = dmalloc(&b->visits_d, C.max_sessions)) || (rc = hmalloc(&b->sess_h, C.max_sessions)) ||
             (rc = hmalloc(&b->out_h, C.max_sessions)) || (rc = hmalloc(&b->jobs_h, C.max_sessions)) ||
@@ -405,7 +401,7 @@ void tsi_batch_destroy(tsi_batch *b) {
 int tsi_batch_tune(tsi_batch *b, const tsi_batch_config *cfg) {
     if (!b || !cfg) return TSI_E_ARG;
     if (b->pending) return TSI_E_PENDING;
-    if (cfg->waves_per_simd > b->max_bpc || cfg->trace_blocks_per_cu > 16u || cfg->claim_units > 64u) return TSI_E_ARG;
+    if (cfg->waves_per_simd > b->max_bpc || cfg->claim_units > 64u) return TSI_E_ARG;
     tsi_batch_config &C = b->cfg;
     if (cfg->waves_per_simd) {
         C.waves_per_simd = cfg->waves_per_simd;
@@ -413,11 +409,6 @@ int tsi_batch_tune(tsi_batch *b, const tsi_batch_config *cfg) {
         b->info.waves_per_simd = cfg->waves_per_simd;
     }
     if (cfg->claim_units) C.claim_units = cfg->claim_units;
-    if (cfg->trace_claim_tiles) C.trace_claim_tiles = cfg->trace_claim_tiles;
-    if (cfg->trace_blocks_per_cu) {
-        C.trace_blocks_per_cu = cfg->trace_blocks_per_cu;
-        b->trace_grid = uint32_t(b->num_cu) * cfg->trace_blocks_per_cu;
-    }
     return TSI_OK;
 }
 
@@ -429,8 +420,6 @@ int tsi_batch_info_get(tsi_batch *b, tsi_batch_info *out) {
     b->info.sessions_open = open;
     b->info.max_waves_per_simd = b->max_bpc;
     b->info.claim_units = b->cfg.claim_units;
-    b->info.trace_claim_tiles = b->cfg.trace_claim_tiles;
-    b->info.trace_blocks_per_cu = b->cfg.trace_blocks_per_cu;
     *out = b->info;
     return TSI_OK;
 }
@@ -527,7 +516,7 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
         return b->sessions[reqs[x].session].tape < b->sessions[reqs[y].session].tape;
     });
-    uint32_t njobs = 0, tiles = 0;
+    uint32_t njobs = 0, blocks = 0;
     for (uint32_t j = 0; j < n; ++j) {
         const uint32_t i = b->order[j];
         const tsi_frame_req &r = reqs[i];
@@ -557,9 +546,9 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
             J.t.visits = b->visits_d + njobs;
             J.t.shading = s.p.shading;
             J.t.grad = s.a.grad;
-            J.tiles_x = (r.width + 7u) / 8u;
-            J.tile0 = tiles;
-            tiles += J.tiles_x * ((r.height + 7u) / 8u);
+            J.blocks_x = (r.width + 15u) / 16u;
+            J.block0 = blocks;
+            blocks += J.blocks_x * ((r.height + 15u) / 16u);
             b->job_of[i] = int32_t(njobs);
             s.w = r.width;
             s.h = r.height;
@@ -585,13 +574,9 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     b->traced = njobs > 0;
     if (njobs) {
         HIP_OK(hipMemcpyAsync(b->jobs_d, b->jobs_h, njobs * sizeof(TraceJob), hipMemcpyHostToDevice, b->stream));
-        if (launch_fill_u32(b->trace_claim, 0u, 1u, b->stream) ||
-            launch_fill_u32(reinterpret_cast<uint32_t *>(b->visits_d), 0u, 2u * njobs, b->stream))
-            return TSI_E_HIP;
+        if (launch_fill_u32(reinterpret_cast<uint32_t *>(b->visits_d), 0u, 2u * njobs, b->stream)) return TSI_E_HIP;
         HIP_OK(hipEventRecord(b->ev[2], b->stream));
-        if (launch_trace_batch(b->jobs_d, njobs, tiles, b->cfg.trace_claim_tiles, b->trace_claim, b->trace_grid,
-                               b->stream))
-            return TSI_E_HIP;
+        if (launch_trace_batch(b->jobs_d, njobs, blocks, b->stream)) return TSI_E_HIP;
         HIP_OK(hipMemcpyAsync(b->visits_h, b->visits_d, njobs * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               b->stream));
     }

@@ -60,16 +60,18 @@ struct LevelTally {
 };
 
 // Frontier entries of one list at a level (a list that overflowed, flagged by wave_append,
-// holds list_cap valid entries).
-TSI_DI uint32_t list_entries(const KArgs &a, int level, int which) {
+// holds list_cap valid entries).  KA: KArgs, or KArgs in the constant address space (a table
+// the kernel only reads; its fields are then scalar loads).
+template <class KA>
+TSI_DI uint32_t list_entries(const KA &a, int level, int which) {
     return min(a.cnt->list_count[level][which], a.list_cap);
 }
 
 // Group g (entries 8g .. 8g + 7) of one frontier list at one level: 8 parents x 8 child octants
 // on the wavefront's 64 lanes.  count = list_entries(a, level, REV); stk = the wavefront's
 // saved-operand stack (uniform base; lane l uses column l of each row).
-template <bool REV, class Engine>
-TSI_DI void eval_group(const KArgs &a, int level, float time, uint32_t g, uint32_t count, float2 *stk, Engine &E,
+template <bool REV, class Engine, class KA>
+TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t count, float2 *stk, Engine &E,
                        Slots &Rlo, Slots &Rhi, LevelTally &tl) {
     const int which = REV ? 1 : 0;
     Counters *cnt = a.cnt;

@@ -755,11 +755,11 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     o += "    static constexpr uint32_t kRegions = " + std::to_string(Y.regions) + "u;\n";
     o += "    uint32_t lane, recs;\n    const uint32_t *prec;\n    uint32_t *chbuf;\n    RegionMask mask;\n"
          "    uint32_t variant;   // this wavefront's: 0 full, 1 full + choices, 2 pruned\n";
-    o += "    TSI_DI TapeEngine(const KArgs &a, uint32_t wave, uint32_t l)\n"
+    o += "    template <class KA>\n    TSI_DI TapeEngine(const KA &a, uint32_t wave, uint32_t l)\n"
          "        : lane(l), recs(kTapeRecords), prec(nullptr), variant(0) {\n"
          "        for (uint32_t w = 0; w < kRegionWords; ++w) mask.w[w] = 0xFFFFFFFFu;\n"
          "        chbuf = a.choice ? a.choice + size_t(wave) * kModeWords * 64u + l : nullptr;\n    }\n";
-    o += "    TSI_DI void select(const KArgs &a, int, uint32_t, bool pruned_ok, const uint32_t *prec_in, uint32_t pidx,\n"
+    o += "    template <class KA>\n    TSI_DI void select(const KA &a, int, uint32_t, bool pruned_ok, const uint32_t *prec_in, uint32_t pidx,\n"
          "                       bool need_choices) {\n"
          "        prec = nullptr;\n"
          "        variant = 0;\n"
@@ -785,8 +785,8 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
              "        }\n";
     o += "    }\n";
     const std::string fcall = "(x, y, z, t, stk, lane, prec, a.prune_cap, chbuf, mask)";
-    o += "    template <bool PUSH>\n"
-         "    TSI_DI uint32_t forward(const KArgs &a, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n";
+    o += "    template <bool PUSH, class KA>\n"
+         "    TSI_DI uint32_t forward(const KA &a, Iv x, Iv y, Iv z, Iv t, float2 *stk, Iv &field, Slots &, Slots &) {\n";
     if (prune)
         o += "        const FwdOut r = variant == 2u ? tape_forward<PUSH>" + fcall + "\n"
              "                       : variant == 1u ? tape_forward_choices<PUSH>" + fcall + "\n"
@@ -796,21 +796,21 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     o += "        field = r.field;\n        recs = r.records;\n        return r.cost;\n    }\n";
     const std::string rcall = "(stk, lane, prec, a.prune_cap, mask)";
     if (rev_kernel) {
-        o += "    TSI_DI uint32_t reverse(const KArgs &a, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n";
+        o += "    template <class KA>\n    TSI_DI uint32_t reverse(const KA &a, const float2 *stk, Iv sens[4], Slots &, Slots &) {\n";
         if (prune)
             o += "        const RevOut r = variant == 2u ? tape_reverse" + rcall + " : tape_reverse_full" + rcall + ";\n";
         else o += "        const RevOut r = tape_reverse" + rcall + ";\n";
         o += "        sens[0] = r.s0; sens[1] = r.s1; sens[2] = r.s2; sens[3] = r.s3;\n        return r.ops;\n    }\n";
     } else {
-        o += "    TSI_DI uint32_t reverse(const KArgs &, const float2 *, Iv sens[4], Slots &, Slots &) {\n"
+        o += "    template <class KA>\n    TSI_DI uint32_t reverse(const KA &, const float2 *, Iv sens[4], Slots &, Slots &) {\n"
              "        for (int k = 0; k < 4; ++k) sens[k] = I_FROM(0.0f);\n        return 0u;\n    }\n";
     }
     o += "    TSI_DI uint32_t records() const { return recs; }\n";
     o += "    TSI_DI bool choices_recorded() const { return variant != 0u; }\n";
     if (prune)
-        o += "    TSI_DI void liveness(const KArgs &a, uint32_t *rec) { tape_liveness(chbuf, rec, a.prune_cap, mask); }\n";
+        o += "    template <class KA>\n    TSI_DI void liveness(const KA &a, uint32_t *rec) { tape_liveness(chbuf, rec, a.prune_cap, mask); }\n";
     else
-        o += "    TSI_DI void liveness(const KArgs &, uint32_t *) {}\n";
+        o += "    template <class KA>\n    TSI_DI void liveness(const KA &, uint32_t *) {}\n";
     o += "};\n\n}  // namespace tsi\n\n";
     o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") +
          (rev_kernel ? "tsi_tape_eval_rev" : "tsi_tape_eval_fwd") +
@@ -871,15 +871,15 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
     }
     // the tape of a unit is the session's, uniform over the wavefront
     o += "struct BatchTapes {\n"
-         "    template <bool REV, class E>\n"
-         "    static TSI_DI void run(const KArgs &a, int level, float time, uint32_t g, uint32_t count, float2 *stk,\n"
+         "    template <bool REV, class E, class KA>\n"
+         "    static TSI_DI void run(const KA &a, int level, float time, uint32_t g, uint32_t count, float2 *stk,\n"
          "                           LevelTally &tl) {\n"
          "        E eng(a, 0u, lane_id());\n"
          "        Slots Rlo, Rhi;\n"
          "        eval_group<REV, E>(a, level, time, g, count, stk, eng, Rlo, Rhi, tl);\n"
          "    }\n"
-         "    template <bool REV>\n"
-         "    static TSI_DI void group(uint32_t tape, const KArgs &a, int level, float time, uint32_t g, uint32_t count,\n"
+         "    template <bool REV, class KA>\n"
+         "    static TSI_DI void group(uint32_t tape, const KA &a, int level, float time, uint32_t g, uint32_t count,\n"
          "                             float2 *stk, LevelTally &tl) {\n"
          "        switch (tape) {\n";
     for (size_t i = 0; i < tapes.size(); ++i) {

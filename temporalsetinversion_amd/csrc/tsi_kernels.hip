@@ -390,40 +390,44 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
 }
 
-// The frames of a batch in one persistent launch: wavefronts take 8x8 pixel tiles of any frame
-// from a counter (claim_tiles per atomic) until none is left.  Frame j's tiles are
-// [jobs[j].tile0, jobs[j + 1].tile0), row-major over ceil(width / 8) tiles per row.
-__global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles,
-                                                     uint32_t claim_tiles, uint32_t *claim) {
+// The frames of a batch in one launch: block i is a 16x16-pixel block of the frame whose blocks
+// [jobs[j].block0, jobs[j + 1].block0) hold it, row-major over ceil(width / 16) blocks per row,
+// and its wavefronts take the 8x8 quadrants, as k_trace's do (a persistent grid taking tiles
+// from a counter measured slower: a frame's slowest tiles then ran one after another in a few
+// wavefronts, where the hardware dispatcher starts each block on the first free slot).
+__global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint32_t njobs) {
     __shared__ uint32_t stack_lds[16 * 256];
-    const uint32_t lane = lane_id();
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(claim, claim_tiles);
-        base = __builtin_amdgcn_readfirstlane(bcast(base, 0));
-        if (base >= total_tiles) break;
-        const uint32_t end = min(base + claim_tiles, total_tiles);
-        for (uint32_t tile = base; tile < end; ++tile) {
-            uint32_t lo = 0, hi = njobs;   // last job with tile0 <= tile
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
-            }
-            const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
-            const TraceJob &J = jobs[j];
-            KArgs a{};
-            a.st = const_cast<float4 *>(J.st);
-            a.blo = const_cast<float4 *>(J.blo);
-            a.bhi = const_cast<float4 *>(J.bhi);
-            const uint32_t local = tile - J.tile0;
-            const uint32_t x = (local % J.tiles_x) * 8u + (lane & 7u);
-            const uint32_t y = (local / J.tiles_x) * 8u + (lane >> 3);
-            uint32_t visits = 0;
-            if (x < J.t.width && y < J.t.height) trace_pixel(a, J.t, x, y, stack_lds + threadIdx.x, visits);
-            const uint64_t v = wave_sum64(visits);
-            if (lane == 0 && v) atomicAdd((unsigned long long *)J.t.visits, (unsigned long long)v);
-        }
+    const uint32_t blk = blockIdx.x;
+    uint32_t lo = 0, hi = njobs;   // last job with block0 <= blk
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (as_const(jobs[mid]).block0 <= blk) lo = mid; else hi = mid;
     }
+    // the job table is written before the launch and only read here: scalar loads (as_const)
+    const auto &J = as_const(jobs[__builtin_amdgcn_readfirstlane(lo)]);
+    KArgs a{};
+    a.st = const_cast<float4 *>(J.st);
+    a.blo = const_cast<float4 *>(J.blo);
+    a.bhi = const_cast<float4 *>(J.bhi);
+    TraceArgs t;
+    for (int k = 0; k < 16; ++k) t.view[k] = J.t.view[k];
+    t.width = J.t.width;
+    t.height = J.t.height;
+    t.time = J.t.time;
+    t.zdist = J.t.zdist;
+    t.rgba = J.t.rgba;
+    t.depth = J.t.depth;
+    t.visits = J.t.visits;
+    t.shading = J.t.shading;
+    t.grad = J.t.grad;
+    const uint32_t local = blk - J.block0;
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    const uint32_t x = (local % J.blocks_x) * 16u + (w & 1u) * 8u + (l & 7u);
+    const uint32_t y = (local / J.blocks_x) * 16u + (w >> 1) * 8u + (l >> 3);
+    uint32_t visits = 0;
+    if (x < t.width && y < t.height) trace_pixel(a, t, x, y, stack_lds + threadIdx.x, visits);
+    const uint64_t v = wave_sum64(visits);
+    if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
 }
 
 // RGBA32F -> RGBA8 as the reference's frame dump converts it (src/main.rs:709-711, image 0.24
@@ -515,9 +519,8 @@ int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles, uint32_t claim_tiles,
-                       uint32_t *claim, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_batch, dim3(grid), dim3(256), 0, s, jobs, njobs, total_tiles, claim_tiles, claim);
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_batch, dim3(total_blocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

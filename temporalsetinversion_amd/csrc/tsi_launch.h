@@ -31,11 +31,10 @@ int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s);
 struct TraceJob {
     const float4 *st, *blo, *bhi, *grad;
     TraceArgs t;
-    uint32_t tiles_x;   // ceil(width / 8)
-    uint32_t tile0;     // the frame's first tile in the batch
+    uint32_t blocks_x;  // ceil(width / 16): 16x16-pixel blocks per row
+    uint32_t block0;    // the frame's first block in the launch
 };
-int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_tiles, uint32_t claim_tiles,
-                       uint32_t *claim, uint32_t grid, hipStream_t s);
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, hipStream_t s);
 // The canonical node records of an octree (tsi_read_nodes): every node reachable from the root
 // through nodes with children, keyed by (depth, i, j, k).  Synchronous device reads.
 int read_octree(const float4 *st_dev, const float4 *grad_dev, const Counters *cnt_dev, uint64_t nodes, tsi_node *out,

@@ -264,9 +264,10 @@ void release_jit(tsi_ctx *c) {
 }
 
 // Compiles (or finds cached) the tape's two level kernels and loads them on this device.
-// The root box of the reference (main.rs) and a t-range around the frame loop's [0, 1]: the tape
-// kernels are compiled for it, and recompiled without that assumption if a frame leaves it.
-const JitDomain kDefaultDomain{{-5.0f, -5.0f, -5.0f, 5.0f, 5.0f, 5.0f}, -2.0f, 3.0f};
+// The root box of the reference (main.rs) and a t-range around the frame loop's times (about a
+// minute of frames at 1/60): the tape kernels are compiled for it, and recompiled without that
+// assumption if a frame leaves it.
+const JitDomain kDefaultDomain{{-5.0f, -5.0f, -5.0f, 5.0f, 5.0f, 5.0f}, -2.0f, 64.0f};
 
 // The generated module addresses no stack row, liveness-record word or choice column beyond what
 // alloc_prune and tsi_upload_tape allocate (a module that would is refused, not run).

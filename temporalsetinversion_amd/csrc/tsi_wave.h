@@ -43,6 +43,13 @@ TSI_DI uint64_t wave_sum64(uint64_t v) {
     return v;
 }
 
+// A struct the kernel only reads (a table written before the launch), seen through the constant
+// address space: reading a field is a scalar load, as for a launch's own arguments.
+template <class T>
+TSI_DI const __attribute__((address_space(4))) T &as_const(const T &x) {
+    return *(const __attribute__((address_space(4))) T *)&x;
+}
+
 // node state (st[].w)
 TSI_DI uint32_t meta_of(float4 s) { return __float_as_uint(s.w); }
 constexpr uint32_t kFastBit = 0x80000000u;

@@ -2,7 +2,7 @@
 tsi_precompile_tape on the CPU (hiprtc needs no GPU).
 
 fix (compute.wgsl:3462-3472) clamps to +-1e6; on the default domain (cells in [-5,5]^3,
-t-windows in [-2, 3]) a value is left unfixed only when the interval rules, evaluated over the
+t-windows in [-2, 64]) a value is left unfixed only when the interval rules, evaluated over the
 whole domain, keep it inside the clamp.  Closed forms pin the count: x^(2^k) over [-5, 5] is at
 most 5^(2^k) = 25, 625, 390625, 1.5e11, ... so the first three squarings are fix-free and every
 record from the fourth on keeps its fix; 1/x (a division by an interval containing 0) is never
