@@ -203,6 +203,28 @@ def summarize_profile(args) -> None:
     print("\n".join(lines))
 
 
+def summarize_kernels(args) -> None:
+    """Per-kernel dispatch counts and durations of a rocprofv3 kernel trace (the batched form's
+    profile summary: the persistent set-inversion launch and the trace launch of each submit)."""
+    rows = _csv_rows(args.summarize_kernels)
+    per = {}
+    for r in rows:
+        name = _kernel(r["Kernel_Name"])
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        per.setdefault(name, []).append(d)
+    total = sum(sum(v) for v in per.values())
+    R = "$R"
+    lines = [f"# Kernel time: {args.summary_title}", "#",
+             "# command (from /tmp on the GPU host, repo at $R, <d> = an output directory):",
+             f"#   rocprofv3 --kernel-trace --stats -f csv -d <d> -o kt -- python3 {R}/bench.py {args.summary_cmd}",
+             f"#   python3 bench.py --summarize-kernels <d>/kt_kernel_trace.csv --summary-title ... --summary-cmd ...",
+             "", f"{'kernel':<44} {'dispatches':>10} {'total_ms':>10} {'avg_us':>11} {'min_us':>11} {'max_us':>11} {'share':>7}"]
+    for name, v in sorted(per.items(), key=lambda x: -sum(x[1])):
+        lines.append(f"{name[:44]:<44} {len(v):>10} {sum(v) / 1e3:>10.3f} {sum(v) / len(v):>11.1f} {min(v):>11.1f} "
+                     f"{max(v):>11.1f} {100 * sum(v) / total:>6.1f}%")
+    print("\n".join(lines))
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -579,6 +601,10 @@ def main():
     ap.add_argument("--copy-fetch", metavar="DB")
     ap.add_argument("--copy-write", metavar="DB")
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--summarize-kernels", metavar="KERNEL_TRACE_CSV",
+                    help="instead of the benchmark: per-kernel summary of a rocprofv3 kernel trace")
+    ap.add_argument("--summary-title", default="")
+    ap.add_argument("--summary-cmd", default="")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
     ap.add_argument("--no-served", action="store_true", help="skip the served form")
     ap.add_argument("--no-tune", action="store_true", help="batched form: keep the default launch shape")
@@ -595,6 +621,9 @@ def main():
     args = ap.parse_args()
     if args.summarize:
         summarize_profile(args)
+        return
+    if args.summarize_kernels:
+        summarize_kernels(args)
         return
     if args.calibrate_copy_mib:
         import torch

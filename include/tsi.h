@@ -276,7 +276,7 @@ typedef struct tsi_batch tsi_batch;
 typedef struct {
     uint64_t arena_bytes;         /* admission budget: sessions' pools, lists and frames */
     uint32_t max_sessions;        /* sessions open at once (0 = 1024, the most) */
-    uint32_t waves_per_simd;      /* persistent grid: 1..3 wavefronts per SIMD (0 = 2) */
+    uint32_t waves_per_simd;      /* persistent grid: 1..4 wavefronts per SIMD (0 = 2) */
     uint32_t claim_units;         /* work units (8 frontier entries each) per queue claim (0 = 1) */
 } tsi_batch_config;
 

@@ -126,8 +126,8 @@ TSI_DI uint64_t dealloc_group(const KA &a, int step, uint32_t g) {
 }
 
 // The persistent kernel's body.  Tapes::group<REV>(tape, a, level, time, g, count, stk, tally)
-// runs eval_group with the tape's engine.  One block per CU, of 4, 8 or 12 wavefronts (one to
-// three per SIMD).
+// runs eval_group with the tape's engine.  One block per CU, of 4, 8, 12 or 16 wavefronts (one
+// to four per SIMD).
 template <class Tapes>
 TSI_DI void batch_run(const BatchArgs &b) {
     // prefix sums of the phase's units over sessions, in queue order: [0, n) reverse units of

@@ -342,14 +342,14 @@ int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *le
         if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         b->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         // one block per CU of 4 x waves_per_simd wavefronts: the most that fit a CU is how
-        // many 256-thread blocks would (each is one wavefront per SIMD), at most 3 (the kernel's
+        // many 256-thread blocks would (each is one wavefront per SIMD), at most 4 (the kernel's
         // launch bound); the stack is sized for that (tsi_batch_tune may change the block)
         int per_cu = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 256, 0) != hipSuccess || per_cu < 1) {
             rc = TSI_E_HIP;
             break;
         }
-        b->max_bpc = std::min<uint32_t>(3u, uint32_t(per_cu));
+        b->max_bpc = std::min<uint32_t>(4u, uint32_t(per_cu));
         const uint32_t bpc = std::min<uint32_t>(C.waves_per_simd, b->max_bpc);
         b->wps = bpc;
         b->grid = uint32_t(b->num_cu);

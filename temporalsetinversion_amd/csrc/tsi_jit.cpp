@@ -890,8 +890,8 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
              "            break;\n";
     }
     o += "        default: break;\n        }\n    }\n};\n}  // namespace tsi\n\n";
-    // one block per CU of up to 12 wavefronts (3 per SIMD)
-    o += "extern \"C\" __global__ void __launch_bounds__(768) tsi_batch_run(tsi::BatchArgs b) {\n"
+    // one block per CU of up to 16 wavefronts (4 per SIMD)
+    o += "extern \"C\" __global__ void __launch_bounds__(1024) tsi_batch_run(tsi::BatchArgs b) {\n"
          "    tsi::batch_run<tsi::BatchTapes>(b);\n}\n";
     return o;
 }

@@ -21,8 +21,8 @@ import json
 import time
 
 KNOBS = {
-    "waves_per_simd": (1, 2, 3),
-    "claim_units": (1, 2, 4, 8),
+    "waves_per_simd": (1, 2, 3, 4),
+    "claim_units": (1, 2, 4, 8, 16),
 }
 
 
