@@ -74,6 +74,17 @@ uint32_t region_records(bool prune) {
     return g ? g : (prune ? 16u : 32u);
 }
 
+// Regions ahead whose saved-operand loads the reverse pass of an unpruned module has in flight
+// ($TSI_JIT_PREFETCH, default 1: the next region's loads are issued before this region runs).
+uint32_t prefetch_regions() {
+    static const uint32_t d = [] {
+        const char *e = std::getenv("TSI_JIT_PREFETCH");
+        const long v = e ? std::strtol(e, nullptr, 10) : 0;
+        return v >= 1 && v <= 8 ? uint32_t(v) : 1u;
+    }();
+    return d;
+}
+
 std::string V(uint32_t j) { return "v" + std::to_string(j); }
 std::string S(uint32_t j) { return "s" + std::to_string(j); }
 // row pos of the wavefront's stack, this lane's column (uniform row base + lane offset)
@@ -530,19 +541,32 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         // inside one, a value would merge with its old value after the block, and the copy that
         // merge needs would wait for the load there.  Regions that a pruned program skips get
         // their operands loaded all the same (the rows exist; the values are then unused).
-        if (lean) emit_loads(n, region_lo(n));   // the top region's operands
-        else if (!pr) {
+        // lean: the regions from the top, and the loads of the region D below the one about to
+        // run, except what the D regions from this one on read (those are in registers already)
+        std::vector<std::pair<uint32_t, uint32_t>> regs;   // (hi, lo)
+        for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) regs.push_back({hi, region_lo(hi)});
+        const uint32_t D = prefetch_regions();
+        auto lean_loads = [&](size_t i) {   // loads of region i, skipping regions i-D .. i-1
+            std::vector<char> vskip(n, 0), wskip(code_words + 1, 0), vn, wn;
+            for (size_t q = i >= D ? i - D : 0; q < i; ++q) {
+                region_needs(regs[q].first, regs[q].second, vn, wn);
+                for (uint32_t k = 0; k < n; ++k) vskip[k] |= vn[k];
+                for (uint32_t w = 0; w <= code_words; ++w) wskip[w] |= wn[w];
+            }
+            emit_loads(regs[i].first, regs[i].second, &vskip, &wskip);
+        };
+        if (lean) {   // the top regions' operands
+            for (size_t i = 0; i < D && i < regs.size(); ++i) lean_loads(i);
+        } else if (!pr) {
             o += open_region(0);
             emit_loads(n, region_lo(n));
             o += close_region();
         }
-        for (uint32_t hi = n; hi > 4u; hi = region_lo(hi)) {
+        size_t ri = 0;
+        for (uint32_t hi = n; hi > 4u; hi = region_lo(hi), ++ri) {
             const uint32_t lo = region_lo(hi), r = region_of(lo);
-            if (lean && lo > 4u) {   // the next region's operands go in flight before this region runs
-                std::vector<char> vnow, wnow;
-                region_needs(hi, lo, vnow, wnow);
-                emit_loads(lo, region_lo(lo), &vnow, &wnow);
-            }
+            // the operands of the region D below go in flight before this region runs
+            if (lean && ri + D < regs.size()) lean_loads(ri + D);
             o += open_region(r);
             if (!lean) {
                 if (pr) {
