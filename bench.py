@@ -337,6 +337,32 @@ def _c1_worker(job):
     return frames, cells
 
 
+def gpu_c1(T, frames: int = 120) -> dict:
+    """The reference's CPU configuration on the plain GPU path, as cpu_baseline_c1 runs it: 8 moving
+    spheres, localsubdiv 6, 256x256, a blocking set inversion and trace per frame at t_k = k/60."""
+    from temporalsetinversion_amd import scenes
+
+    st = T.State(T.tape_from_ll(scenes.spheres(0)), width=256, height=256, localsubdiv=6, voxel_capacity=1 << 20)
+    view = T.diagonal_camera()
+    st.reset()
+    for k in range(10):   # warm-up
+        st.update(float(np.float32(k / 60.0)), view)
+        st.set_inversion()
+        st.trace(pixels=False)
+        st.enqueue_root()
+    cells = 0
+    t0 = time.perf_counter()
+    for k in range(10, 10 + frames):
+        st.update(float(np.float32(k / 60.0)), view)
+        cells += st.set_inversion().stats["cell_evals"]
+        st.trace(pixels=False)
+        st.enqueue_root()
+    wall = time.perf_counter() - t0
+    st.close()
+    return {"frames_per_s": frames / wall, "cell_evals_per_s": cells / wall, "frames": frames,
+            "config": "spheres8_L6_256 (as cpu_baseline_c1), blocking set inversion + trace per frame"}
+
+
 def full_reevaluation(st, frames: int, t0_frame: int):
     """Set inversion of `frames` frames each from an empty octree (tsi_reset before the frame: every
     cell new), device ms per frame; the temporal method's comparison of the reference."""
@@ -800,6 +826,7 @@ def main():
     }
     if tvf is not None:
         out["temporal_vs_full"] = tvf
+    out["gpu_c1"] = gpu_c1(T)
     if not args.no_batched:
         print("batched form", file=sys.stderr, flush=True)
         out["batched"] = batched_form(T, args)
