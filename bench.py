@@ -363,6 +363,74 @@ def gpu_c1(T, frames: int = 120) -> dict:
             "config": "spheres8_L6_256 (as cpu_baseline_c1), blocking set inversion + trace per frame"}
 
 
+def cpu_cinematic(budget_s: float) -> dict:
+    """The oracle's cinematic trace (the path tracer of get_color_supersample / get_color_single)
+    on one host core over a bounded sample: rows of the spheres8_L6_256 frame at t = 10/60, 8 rows
+    at a time, until the budget is spent."""
+    import oracle
+    import temporalsetinversion_amd as T
+    from temporalsetinversion_amd import scenes
+
+    fr = oracle.Frame(oracle.parse_ll(scenes.spheres(0)), localsubdiv=6, voxel_capacity=1 << 20,
+                      shading=T._lib.TSI_SHADE_CINEMATIC)
+    for k in range(11):
+        fr.set_inversion(float(np.float32(k / 60.0)))
+        if k < 10:
+            fr.enqueue_root()
+    view = np.array(T.diagonal_camera(), np.float32)
+    zd = float(T.camera_zdist(256))
+    rows = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and rows < 256:
+        fr.trace(view, 256, 256, float(np.float32(10 / 60.0)), zd, row0=rows, rows=8)
+        rows += 8
+    wall = time.perf_counter() - t0
+    fr.close()
+    return {"value": rows * 256 / wall, "unit": "pixels/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} rows of the spheres8_L6_256 frame at t = 10/60 (8 jittered samples, 2 bounces)",
+            "seconds": wall}
+
+
+def cinematic_form(T, args) -> dict:
+    """The cinematic path tracer (TSI_SHADE_CINEMATIC) after the same set inversion as the plain
+    frames: the reference's CPU configuration (spheres8_L6_256) and the headline workload.  Device
+    time of the trace launch, node visits (octree child reads) per frame, and the algorithmic bytes
+    of the trace (40 B per node visit + 20 B per pixel) over its time."""
+    from temporalsetinversion_amd import scenes
+
+    out = {}
+    cases = [("spheres8_L6_256", WORKLOADS["spheres8_L6_256"], T.diagonal_camera(), 20),
+             (args.workload, WORKLOADS[args.workload], None, 3)]
+    for name, w, view, frames in cases:
+        tape = T.tape_from_ll(scenes.SCENES[w["scene"]](w["seed"]))
+        st = T.State(tape, width=w["width"], height=w["height"], localsubdiv=w["localsubdiv"], delta=0.01,
+                     point_sample=w["point_sample"], swept_volume=w["swept_volume"],
+                     lipschitz_corrected=w["lipschitz_corrected"],
+                     voxel_capacity=1 << (20 if w["localsubdiv"] <= 6 else 27), prune=False,
+                     shading=T._lib.TSI_SHADE_CINEMATIC)
+        st.reset()
+        ms = visits = 0.0
+        for k in range(3 + frames):
+            st.update(float(np.float32(k / 60.0)), view)
+            st.set_inversion()
+            st.trace(pixels=False)
+            st.enqueue_root()
+            if k >= 3:
+                m, v = st.last_trace()
+                ms += m
+                visits += v
+        st.close()
+        px = w["width"] * w["height"]
+        ms /= frames
+        visits /= frames
+        gbs = (visits * 40 + px * 20) / (ms * 1e-3) / 1e9
+        out[name] = {"trace_ms_per_frame": ms, "pixels_per_s": px / (ms * 1e-3), "node_visits_per_frame": visits,
+                     "node_visits_per_pixel": visits / px, "frame": f"{w['width']}x{w['height']}",
+                     "roofline": {"bound": "hbm", "kernel": "k_trace<true>", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS}, "frames": frames}
+    return out
+
+
 def full_reevaluation(st, frames: int, t0_frame: int):
     """Set inversion of `frames` frames each from an empty octree (tsi_reset before the frame: every
     cell new), device ms per frame; the temporal method's comparison of the reference."""
@@ -633,6 +701,7 @@ def main():
     ap.add_argument("--summary-cmd", default="")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
     ap.add_argument("--no-served", action="store_true", help="skip the served form")
+    ap.add_argument("--no-cinematic", action="store_true", help="skip the cinematic path tracer")
     ap.add_argument("--no-tune", action="store_true", help="batched form: keep the default launch shape")
     ap.add_argument("--batch-requests", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -671,13 +740,14 @@ def main():
 
     w = WORKLOADS[args.workload]
     ll = scenes.SCENES[w["scene"]](w["seed"])
-    cpu = cpu_c1 = None
+    cpu = cpu_c1 = cpu_cin = None
     if not args.no_cpu:   # before the GPU is touched (worker processes are forked)
         workers = max(1, min(16, os.cpu_count() or 1))
         print(f"cpu baseline: {workers} workers x {args.cpu_budget:.0f} s (same workload), then the spheres "
               f"configuration", file=sys.stderr, flush=True)
         cpu = cpu_baseline(ll, w, args.cpu_budget, workers)
         cpu_c1 = cpu_baseline_c1(min(args.cpu_budget, 6.0), workers)
+        cpu_cin = None if args.no_cinematic else cpu_cinematic(min(args.cpu_budget, 6.0))
 
     import temporalsetinversion_amd as T
 
@@ -833,9 +903,14 @@ def main():
     if not args.no_served:
         print("served form", file=sys.stderr, flush=True)
         out["served"] = served_form(T, args)
+    if not args.no_cinematic:
+        print("cinematic path tracer", file=sys.stderr, flush=True)
+        out["cinematic"] = cinematic_form(T, args)
     if cpu is not None:
         out["cpu_baseline"] = cpu
         out["cpu_baseline_c1"] = cpu_c1
+        if cpu_cin is not None:
+            out["cpu_baseline_cinematic"] = cpu_cin
     print(json.dumps(out), flush=True)
 
 

@@ -74,15 +74,24 @@ typedef struct {
     uint32_t prune_live_max;      /* compiled engine: a wavefront runs its pruned code when at
                                      most this percentage of the tape's regions is live for it
                                      (0 = 75; 100 = whenever its parents' records apply) */
-    uint32_t shading;             /* TSI_SHADE_* (0: the reference's default colouring) */
+    uint32_t shading;             /* TSI_SHADE_* (0: the reference's default colouring; other
+                                     bits than TSI_SHADE_* give TSI_E_ARG) */
 } tsi_params;
 
-/* compute.wgsl:3503-3504 normal_viz / fast_normals as runtime values.  TSI_SHADE_NORMALS: hits
- * show normalize(normal) * 0.5 + 0.5, misses (10, 10, 10, 0) before the 0.45 gamma, no octree
- * darkening (4265, 4333-4377, 4510-4514).  With TSI_SHADE_GRADIENT as well (fast_normals = false)
- * the normal is the voxel's reverse-pass gradient (dx, dy, dz) and every cell takes the reverse
- * pass (3780); without it, it is finite_diff_normal's value for a tape scene, (1, 1, 1). */
-enum { TSI_SHADE_NORMALS = 1u, TSI_SHADE_GRADIENT = 2u };
+/* compute.wgsl:3499, 3503-3504 normal_viz / fast_normals / cinematic as runtime values.
+ * TSI_SHADE_NORMALS: hits show normalize(normal) * 0.5 + 0.5, misses (10, 10, 10, 0) before the
+ * 0.45 gamma, no octree darkening (4265, 4333-4377, 4510-4514).  With TSI_SHADE_GRADIENT as well
+ * (fast_normals = false) the normal is the voxel's reverse-pass gradient (dx, dy, dz) and every
+ * cell takes the reverse pass (3780); without it, it is finite_diff_normal's value for a tape
+ * scene, (1, 1, 1).  TSI_SHADE_CINEMATIC (cinematic = true): the path tracer of
+ * get_color_supersample / get_color_single (4440-4523, 4620-4648): 8 jittered samples per pixel,
+ * two bounces drawn from a cosine-power lobe around the hit normal (the normal as above; every
+ * cell takes the reverse pass when TSI_SHADE_GRADIENT is set too), a sun shadow ray per bounce,
+ * the sunsky background (520-640) for rays that leave after the first bounce, the mean through
+ * LinearToSRGB and the 0.45 gamma; alpha is the fraction of samples whose first ray hit.  The
+ * lighting is the scene-independent one of 4446-4452.  The depth output is the first sample's
+ * first ray.  Tracing costs up to 32 rays per pixel. */
+enum { TSI_SHADE_NORMALS = 1u, TSI_SHADE_GRADIENT = 2u, TSI_SHADE_CINEMATIC = 4u };
 
 enum {
     /* Evaluate the full tape in every cell (the reference's behaviour).  Without it, a cell

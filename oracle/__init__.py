@@ -67,12 +67,20 @@ def lib():
         L.orc_parse_ll.argtypes = [C.c_char_p, C.c_size_t, C.c_float, C.POINTER(Instr), C.c_uint32,
                                    C.POINTER(C.c_uint32)]
         L.orc_parse_ll.restype = C.c_int
-        for name in ("orc_det_sinf", "orc_det_expf", "orc_det_logf", "orc_det_pow045"):
+        for name in ("orc_det_sinf", "orc_det_expf", "orc_det_logf", "orc_det_pow045", "orc_det_cosf",
+                     "orc_det_acosf", "orc_linear_to_srgb"):
             getattr(L, name).argtypes = [C.c_float]
             getattr(L, name).restype = C.c_float
-        for name in ("orc_det_atan2f", "orc_wmin", "orc_wmax"):
+        for name in ("orc_det_atan2f", "orc_wmin", "orc_wmax", "orc_det_powf"):
             getattr(L, name).argtypes = [C.c_float, C.c_float]
             getattr(L, name).restype = C.c_float
+        for name in ("orc_frand", "orc_frand0to1"):
+            getattr(L, name).argtypes = [C.POINTER(C.c_uint32)]
+            getattr(L, name).restype = C.c_float
+        L.orc_sample_biased.argtypes = [C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+        L.orc_sample_biased.restype = None
+        L.orc_sunsky.argtypes = [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.orc_sunsky.restype = None
         L.orc_det_powi.argtypes = [C.c_float, C.c_int32]
         L.orc_det_powi.restype = C.c_float
         L.orc_forward_op.argtypes = [C.c_uint32, C.c_uint32, Iv, Iv, C.c_float]
@@ -153,6 +161,38 @@ def prune(tape: np.ndarray, vals: np.ndarray):
     choice = np.zeros(n, np.uint8)
     lib().orc_prune(p, n, v.ctypes.data_as(C.POINTER(Iv)), live.ctypes.data, choice.ctypes.data)
     return live, choice
+
+
+def frand_seq(seed: int, n: int, unit: bool = False):
+    """n draws of frand (or frand0to1 with unit=True) from `seed`; returns (values, states after
+    each draw)."""
+    st = C.c_uint32(seed & 0xFFFFFFFF)
+    f = lib().orc_frand0to1 if unit else lib().orc_frand
+    vals = np.empty(n, np.float32)
+    states = np.empty(n, np.uint32)
+    for i in range(n):
+        vals[i] = f(C.byref(st))
+        states[i] = st.value
+    return vals, states
+
+
+def sample_biased(d, power: float, seed: int, n: int) -> np.ndarray:
+    """n successive getSampleBiased(d, power) draws from `seed` (each takes two frand0to1)."""
+    st = C.c_uint32(seed & 0xFFFFFFFF)
+    dv = (C.c_float * 3)(*[float(x) for x in d])
+    out = (C.c_float * 3)()
+    res = np.empty((n, 3), np.float32)
+    for i in range(n):
+        lib().orc_sample_biased(dv, np.float32(power), C.byref(st), out)
+        res[i] = out[:]
+    return res
+
+
+def sunsky(view, sun_pos=(-0.1, 0.7)) -> np.ndarray:
+    v = (C.c_float * 3)(*[float(x) for x in view])
+    out = (C.c_float * 3)()
+    lib().orc_sunsky(v, np.float32(sun_pos[0]), np.float32(sun_pos[1]), out)
+    return np.array(out[:], np.float32)
 
 
 class Frame:

@@ -14,7 +14,10 @@
  *   atan2: octant reduction, atan t = pi/4 + atan((t-1)/(t+1)) above tan(pi/8), Taylor
  *          series to degree 21;
  *   powi : binary exponentiation (POW_II only ever takes an integer exponent);
- *   pow045: exp(0.45 * log x), the gamma of rt_main.
+ *   pow045: exp(0.45 * log x), the gamma of rt_main;
+ *   cos  : sin's reduction with the quadrant advanced by one;
+ *   pow  : exp(y log x) with pow's special cases at 0 and 1;
+ *   acos : atan2(sqrt((1 - x)(1 + x)), x).
  * Only +, -, *, /, floor, comparisons and exponent-field bit operations are used, in a
  * fixed order with no contraction, so the result is the same on any IEEE binary32
  * machine.  Accuracy is pinned against numpy float64 in tests/test_oracle_math.py.
@@ -67,6 +70,27 @@ float orc_det_sinf(float x)
     float r = ((x - k * PIO2_1) - k * PIO2_2) - k * PIO2_3;
     float kq = k - 4.0f * floorf(k * 0.25f);      /* k mod 4, exact for |k| < 2^24 */
     int q = (int)kq;
+    float r2 = r * r;
+    float s = r + r * r2 * (-1.66666666666666666667e-1f + r2 * (8.33333333333333333333e-3f +
+              r2 * (-1.98412698412698412698e-4f + r2 * 2.75573192239858906526e-6f)));
+    float c = (1.0f - 0.5f * r2) + r2 * r2 * (4.16666666666666666667e-2f + r2 * (-1.38888888888888888889e-3f +
+              r2 * (2.48015873015873015873e-5f + r2 * -2.75573192239858906526e-7f)));
+    switch (q & 3) {
+    case 0: return s;
+    case 1: return c;
+    case 2: return -s;
+    default: return -c;
+    }
+}
+
+/* cos: the sine's reduction with the quadrant advanced by one (no rounding of x + pi/2). */
+float orc_det_cosf(float x)
+{
+    if (!(x - x == 0.0f)) return x - x;
+    float k = floorf(x * TWO_OVER_PI + 0.5f);
+    float r = ((x - k * PIO2_1) - k * PIO2_2) - k * PIO2_3;
+    float kq = k - 4.0f * floorf(k * 0.25f);
+    int q = (int)kq + 1;
     float r2 = r * r;
     float s = r + r * r2 * (-1.66666666666666666667e-1f + r2 * (8.33333333333333333333e-3f +
               r2 * (-1.98412698412698412698e-4f + r2 * 2.75573192239858906526e-6f)));
@@ -176,6 +200,26 @@ float orc_det_powi(float x, int32_t n)
         if (un) b = b * b;
     }
     return n < 0 ? 1.0f / r : r;
+}
+
+/* x^y of the shading formulas (compute.wgsl:4437, 4505, 4603, 446-480): exp(y log x) for
+ * x > 0; 0^y = 0 (y > 0), 1 (y = 0), +inf (y < 0); 1^y = 1; x < 0 -> NaN (pow is undefined
+ * there in GLSL/WGSL). */
+float orc_det_powf(float x, float y)
+{
+    if (x != x || y != y) return x + y;
+    if (x < 0.0f) return u2f(0x7fc00000u);
+    if (x == 1.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : u2f(0x7f800000u));
+    return orc_det_expf(y * orc_det_logf(x));
+}
+
+/* acos x = atan2(sqrt((1 - x)(1 + x)), x) on [-1, 1] (SunIntensity, compute.wgsl:486) */
+float orc_det_acosf(float x)
+{
+    if (x != x) return x;
+    if (x > 1.0f || x < -1.0f) return u2f(0x7fc00000u);
+    return orc_det_atan2f(sqrtf((1.0f - x) * (1.0f + x)), x);
 }
 
 float orc_det_pow045(float x)

@@ -225,7 +225,8 @@ static void process_parent(orc_state *s, uint32_t P, float time, float epsilon, 
             continue;
         iv args[4] = {cx, cy, cz, I(wmin(ct.lo, ct.hi), wmax(ct.lo, ct.hi))};
         int take_derivative = ct.lo != ct.hi ||                           /* 3780 */
-                              ((p->shading & ORC_SHADE_NORMALS) && (p->shading & ORC_SHADE_GRADIENT));
+                              ((p->shading & (ORC_SHADE_NORMALS | ORC_SHADE_CINEMATIC)) &&
+                               (p->shading & ORC_SHADE_GRADIENT));
         iv field = orc_eval_tape(s->tape, s->n, args, take_derivative, s->vals, s->sens,
                                  st ? &st->fwd_ops : NULL, st ? &st->rev_ops : NULL);
         if (st) { st->cell_evals++; if (take_derivative) st->rev_evals++; }
@@ -346,10 +347,15 @@ static inline float mylerp(float a, float b, float t)        /* 4116: mix(a, b, 
 /* normalize (WGSL), as v / length(v) */
 static inline v3 vnormalize(v3 v) { float l = vlength(v); return V(v.x / l, v.y / l, v.z / l); }
 
-/* compute.wgsl:4189-4403 (non-cinematic colour; normal_viz with s->p.shading). */
-static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, float *dist, uint64_t *visits)
+/* compute.wgsl:4189-4403 (normal_viz and cinematic from s->p.shading).  *nrm receives
+ * normal_at_hit: finite_diff_normal's (1, 1, 1) for a tape scene, or the voxel's gradient with
+ * ORC_SHADE_GRADIENT (fast_normals = false, 4333-4342); (0, 0, 0) on a miss. */
+static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, float *dist, uint64_t *visits,
+                      v3 *nrm)
 {
     const int normal_viz = (s->p.shading & ORC_SHADE_NORMALS) != 0;
+    const int cinematic = (s->p.shading & ORC_SHADE_CINEMATIC) != 0;
+    v3 normal_at_hit = V(0.0f, 0.0f, 0.0f);
     const orc_voxel *vox = s->vox;
     uint32_t stack[16];
     uint32_t sp = 0;
@@ -365,6 +371,7 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
     v3 p0 = V(vox[region].x[0], vox[region].y[0], vox[region].z[0]);
     v3 p1 = V(vox[region].x[1], vox[region].y[1], vox[region].z[1]);
     v3 hit = efficient_slabs(p0, p1, o2, inv);
+    if (nrm) *nrm = normal_at_hit;
     if (hit.x == 0.0f) { *col = c; *dist = -1.0f; return; }
     float sms = 0.0f;
     for (uint32_t iter = 0; iter < 5000u; iter++) {
@@ -395,7 +402,7 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
         float mkx = 0.0f, mky = 0.0f, mkz = 0.0f;
         v3 old = o2;
         for (int i = 0; i < 32; i++) {
-            if (!normal_viz && max_component(vsub(p1, p0)) < 1.0f) c = vscale(c, DARKEN);   /* 4265 */
+            if (!cinematic && !normal_viz && max_component(vsub(p1, p0)) < 1.0f) c = vscale(c, DARKEN);   /* 4265 */
             float t = ((vlength(V(mkx * (sd.x - dd.x), mky * (sd.y - dd.y), mkz * (sd.z - dd.z))) + hit.y) + TR_EPS) + sms;
             o2 = vadd(old, vscale(rd, t));
             sms = 0.0f;
@@ -427,16 +434,24 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
                         break;
                     } else {
                         intersected = 1;
+                        normal_at_hit = (s->p.shading & ORC_SHADE_GRADIENT)
+                                            ? V(vox[child].dx, vox[child].dy, vox[child].dz)
+                                            : V(1.0f, 1.0f, 1.0f);
                         if (normal_viz) {                                     /* 4333-4377 */
-                            v3 n = (s->p.shading & ORC_SHADE_GRADIENT) ? V(vox[child].dx, vox[child].dy, vox[child].dz)
-                                                                         : V(1.0f, 1.0f, 1.0f);
+                            v3 n = normal_at_hit;
                             v3 nv = vnormalize(V(n.x * 1.0f, n.y * 1.0f, n.z * 1.0f));
                             c = V(nv.x * 0.5f + 0.5f, nv.y * 0.5f + 0.5f, nv.z * 0.5f + 0.5f);
                             break;
                         }
                         float ts = time - vox[child].t;
                         float tt = ts * 60.0f;
-                        c = V(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
+                        if (cinematic) {   /* movement (0.7, 1e-4, 1e-4), base 0.1: 4346-4374 */
+                            v3 m1 = V(1.0f - 0.7f, 1.0f - 0.0001f, 1.0f - 0.0001f);
+                            v3 b1 = V(1.0f - 0.1f, 1.0f - 0.1f, 1.0f - 0.1f);
+                            c = V(c.x - mylerp(m1.x, b1.x, tt), c.y - mylerp(m1.y, b1.y, tt), c.z - mylerp(m1.z, b1.z, tt));
+                        } else {
+                            c = V(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
+                        }
                         break;
                     }
                 }
@@ -455,9 +470,255 @@ static void trace_ray(const orc_state *s, v3 ro, v3 rd, float time, v3 *col, flo
     if (!intersected) d = -1.0f;
     *col = c;
     *dist = d;
+    if (nrm) *nrm = normal_at_hit;
 #undef PUSH
 #undef POP
 #undef PEAK
+}
+
+/* ---------------- cinematic shading ---------------- */
+
+static inline float vdot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b)
+{
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+#define PI_W 3.14159274101257324f         /* compute.wgsl:2476 PI as f32 */
+
+/* frand0to1 (compute.wgsl:2466-2471): the LCG step seed * 747796405 + 1, then PCG's
+ * RXS-M-XS output permutation, mapped to [0, 1] by / 4294967295.0 (2^32 as f32). */
+float orc_frand0to1(uint32_t *seed)
+{
+    *seed = *seed * 747796405u + 1u;
+    uint32_t w = ((*seed >> ((*seed >> 28u) + 4u)) ^ *seed) * 277803737u;
+    w = (w >> 22u) ^ w;
+    return (float)w / 4294967295.0f;
+}
+
+/* frand (2454-2459): the same draw mapped to [-1, 1] */
+float orc_frand(uint32_t *seed) { return orc_frand0to1(seed) * 2.0f - 1.0f; }
+
+/* ortho (4405-4423): a vector perpendicular to v */
+static v3 ortho(v3 v)
+{
+    return fabsf(v.x) > fabsf(v.z) ? V(-v.y, v.x, 0.0f) : V(0.0f, -v.z, v.y);
+}
+
+/* getSampleBiased (4425-4434): cosine-power lobe of exponent `power` around d */
+static v3 sample_biased(v3 d, float power, uint32_t *seed)
+{
+    v3 dir = vnormalize(d);
+    v3 o1 = vnormalize(ortho(dir));
+    v3 o2 = vnormalize(vcross(dir, o1));
+    float rx = orc_frand0to1(seed);
+    float ry = orc_frand0to1(seed);
+    rx = rx * 2.0f * PI_W;
+    ry = orc_det_powf(ry, 1.0f / (power + 1.0f));
+    float om = sqrtf(1.0f - ry * ry);
+    float cx = orc_det_cosf(rx) * om, sy = orc_det_sinf(rx) * om;
+    return vadd(vadd(vscale(o1, cx), vscale(o2, sy)), vscale(dir, ry));
+}
+
+void orc_sample_biased(const float d[3], float power, uint32_t *seed, float out[3])
+{
+    v3 r = sample_biased(V(d[0], d[1], d[2]), power, seed);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* fromSpherical (494-518) */
+static v3 from_spherical(float px, float py)
+{
+    return V(orc_det_cosf(px) * orc_det_sinf(py), orc_det_sinf(px) * orc_det_sinf(py), orc_det_cosf(py));
+}
+
+/* RayleighPhase (429-436), hgPhase (458-476) with pow(x, 2.0) as x * x */
+static float rayleigh_phase(float c) { return (3.0f / (16.0f * 3.1415927410125732f)) * (1.0f + c * c); }
+static float hg_phase(float c, float g)
+{
+    return (1.0f / (4.0f * 3.1415927410125732f)) *
+           ((1.0f - g * g) / orc_det_powf((1.0f - (2.0f * g) * c) + g * g, 1.5f));
+}
+
+/* SunIntensity (478-492), with the sky's sunIntensity = 1000 (427) */
+static float sun_intensity(float zc)
+{
+    return 1000.0f * wmax(0.0f, 1.0f - orc_det_expf(-((1.6110731611233942f - orc_det_acosf(zc)) / 1.5f)));
+}
+
+/* totalMie (437-456), pow(x, 2.0) as x * x */
+static v3 total_mie(v3 lambda, v3 K, float T)
+{
+    float c = (0.20000000298023224f * T) * 9.99999983775159e-18f;
+    float k = (0.4339999854564667f * c) * 3.1415927410125732f;
+    const float tp = 2.0f * 3.1415927410125732f;
+    v3 q = V(tp / lambda.x, tp / lambda.y, tp / lambda.z);
+    return V((k * (q.x * q.x)) * K.x, (k * (q.y * q.y)) * K.y, (k * (q.z * q.z)) * K.z);
+}
+
+/* sunsky (520-640; Fragmentarium's Sunsky.frag), turbidity = sunSize = 1 (423-424).  The
+ * sun disk it computes is not in its result. */
+static v3 sunsky(v3 view, float spx, float spy)
+{
+    float sadc = orc_det_cosf((1.0f * 3.1415927410125732f) / 180.0f);
+    v3 sd = vnormalize(from_spherical((spx - 0.0f) * 6.28000020980835f, (spy - 0.5f) * 3.140000104904175f));
+    const v3 up = V(0.0f, 0.0f, 1.0f);
+    float cvs = vdot(view, sd);
+    float csu = vdot(sd, up);
+    float cuv = vdot(up, view);
+    if (sadc == 1.0f) return V(1.0f, 0.0f, 0.0f);
+    float sunE = sun_intensity(csu);
+    v3 ray = V(5.176821105123963e-6f, 1.2785348189936485e-5f, 2.8530756026157178e-5f);
+    v3 mie = vscale(total_mie(V(6.800000278417428e-7f, 5.499999815583578e-7f, 4.4999998749517545e-7f),
+                              V(0.6859999895095825f, 0.6779999732971191f, 0.6660000085830688f), 1.0f),
+                    0.004999999888241291f);
+    float zen = wmax(0.0f, cuv);
+    float rol = 8400.0f / zen, mol = 1250.0f / zen;
+    v3 fex = V(orc_det_expf(-((ray.x * rol) + (mie.x * mol))), orc_det_expf(-((ray.y * rol) + (mie.y * mol))),
+               orc_det_expf(-((ray.z * rol) + (mie.z * mol))));
+    float rp = rayleigh_phase(cvs), hp = hg_phase(cvs, 0.800000011920929f);
+    v3 rxe = vscale(ray, rp), mxe = vscale(mie, hp);
+    v3 tl = vadd(ray, mie), lx = vadd(rxe, mxe);
+    v3 se = V(sunE * (lx.x / tl.x), sunE * (lx.y / tl.y), sunE * (lx.z / tl.z));
+    v3 sky = V(se.x * (1.0f - fex.x), se.y * (1.0f - fex.y), se.z * (1.0f - fex.z));
+    float f = wmin(wmax(orc_det_powf(1.0f - vdot(up, sd), 5.0f), 0.0f), 1.0f);
+    v3 m = V(1.0f * (1.0f - f) + orc_det_powf(se.x * fex.x, 0.5f) * f,
+             1.0f * (1.0f - f) + orc_det_powf(se.y * fex.y, 0.5f) * f,
+             1.0f * (1.0f - f) + orc_det_powf(se.z * fex.z, 0.5f) * f);
+    sky = vmul(sky, m);
+    return vscale(sky, 0.009999999776482582f);
+}
+
+void orc_sunsky(const float view[3], float spx, float spy, float out[3])
+{
+    v3 r = sunsky(V(view[0], view[1], view[2]), spx, spy);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* LinearToSRGB (4600-4607), per channel: mix(pow(c, 1/2.4) * 1.055 - 0.055, c * 12.92, c < 0.0031308)
+ * after clamping to [0, 1] */
+float orc_linear_to_srgb(float x)
+{
+    float r = wmin(wmax(x, 0.0f), 1.0f);
+    float a = orc_det_powf(r, 0.416666666666666667f) * 1.055f - 0.055f;
+    float b = r * 12.92f;
+    float t = r < 0.0031308f ? 1.0f : 0.0f;
+    return a * (1.0f - t) + b * t;
+}
+
+typedef struct { float x, y, z, w; } v4;
+
+/* get_color_single (compute.wgsl:4440-4523) for the scene-independent lighting (sunPos (-0.1,
+ * 0.7), sun intensity 3, specular 1, albedo 1, sun size 0.01, background x1: the `scene`
+ * branches of 4455-4468 are the reference's hand-written scenes).  The shadow ray's result is
+ * read only when sunlight > 0 (4501), so it is traced only then.  *depth0: the first ray's
+ * distance. */
+static v4 color_single(const orc_state *s, v3 ro, v3 rd, float time, uint32_t *seed, uint64_t *visits,
+                       float *depth0)
+{
+    const int cinematic = (s->p.shading & ORC_SHADE_CINEMATIC) != 0;
+    const int normal_viz = (s->p.shading & ORC_SHADE_NORMALS) != 0;
+    v3 frm = ro, dir = rd;
+    v3 luminance = V(1.0f, 1.0f, 1.0f), direct = V(0.0f, 0.0f, 0.0f);
+    const float spx = -0.1f, spy = 0.7f;
+    const float sun_i = 3.0f, specular_i = 1.0f, albedo = 1.0f, sunsize = 0.01f, bgmul = 1.0f;
+    int bounces = cinematic ? 2 : 1;
+    for (int i = 0; i < bounces; i++) {
+        v3 c, n;
+        float d;
+        trace_ray(s, frm, dir, time, &c, &d, visits, &n);
+        if (i == 0 && depth0) *depth0 = d;
+        v3 hitpoint = vadd(frm, vscale(dir, d));
+        if (d >= 0.0f) {
+            if (!cinematic) { v4 r = {c.x, c.y, c.z, 1.0f}; return r; }
+            v3 old_rd = dir;
+            dir = sample_biased(n, 3.0f, seed);
+            luminance = vscale(vmul(luminance, c), albedo);
+            frm = vadd(hitpoint, vscale(n, 0.05f));
+            v3 sdn = vnormalize(from_spherical((spx - 0.0f) * 6.28f, (spy - 0.5f) * 3.14f));
+            v3 sundir = V(sdn.x, sdn.z, sdn.y);
+            float j0 = orc_frand(seed);
+            float j1 = orc_frand(seed);
+            float j2 = orc_frand(seed);
+            v3 ssd = vnormalize(vadd(sundir, vscale(V(j0, j1, j2), sunsize)));
+            float sunlight = vdot(n, ssd);
+            if (sunlight > 0.0f) {
+                v3 sc, sn;
+                float sdist;
+                trace_ray(s, frm, ssd, time, &sc, &sdist, visits, &sn);
+                if (sdist < 0.0f) {
+                    direct = vadd(direct, vscale(vscale(luminance, sunlight), sun_i));
+                    v3 half = vnormalize(vadd(V(-old_rd.x, -old_rd.y, -old_rd.z), sundir));
+                    float spec = (100.0f * orc_det_powf(wmax(vdot(n, half), 0.0f), 30.0f)) * specular_i;
+                    direct = vadd(direct, vscale(luminance, spec));
+                }
+            }
+        } else {
+            if (i == 0) {                 /* see_background = false (3506) */
+                if (cinematic || normal_viz) { v4 r = {10.0f, 10.0f, 10.0f, 0.0f}; return r; }
+                v4 r = {c.x, c.y, c.z, 1.0f};
+                return r;
+            }
+            v3 bg = sunsky(V(dir.x, dir.z, dir.y), spx, spy);
+            v3 o = vadd(direct, vmul(luminance, bg));
+            v4 r = {o.x * bgmul, o.y * bgmul, o.z * bgmul, 1.0f * bgmul};
+            return r;
+        }
+    }
+    v4 r = {direct.x, direct.y, direct.z, 1.0f};
+    return r;
+}
+
+/* rt_main -> get_color_supersample with cinematic = true (4620-4648, 4651-4688): seed3 from the
+ * pixel index and the time (4667), totalsamples = 8 rays jittered by frand() * 0.5, the mean
+ * through LinearToSRGB, then the 0.45 gamma; alpha is the mean alpha.  depth: the first
+ * sample's first ray. */
+static int trace_cinematic(const orc_state *s, const float view[16], uint32_t W, uint32_t H, float time,
+                           float zdist, float *rgba, float *depth, uint32_t row0, uint32_t r1,
+                           uint64_t *node_visits)
+{
+    for (uint32_t y = row0; y < r1; y++) {
+        for (uint32_t x = 0; x < W; x++) {
+            uint32_t pi = y * W + x;
+            uint32_t seed = pi + (uint32_t)(12345.0f * (orc_det_sinf(time * 12345.0f) * 0.5f + 0.5f));
+            v3 eye = V(view[12], view[13], view[14]);
+            v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            float d0 = -1.0f;
+            for (int k = 0; k < 8; k++) {
+                float jx = orc_frand(&seed);
+                float jy = orc_frand(&seed);
+                float ox = (jx * 1.0f) * 0.5f, oy = (jy * 1.0f) * 0.5f;
+                float px = (float)x + ox, py = (float)y + oy;
+                /* ray_direction (4590-4594) */
+                float xyx = (px - (float)W / 2.0f) + 0.5f;
+                float xyy = (py - (float)H / 2.0f) + 0.5f;
+                v3 d = V(xyx, -xyy, -zdist);
+                float len = vlength(d);
+                d = V(d.x / len, d.y / len, d.z / len);
+                v3 g = V(((view[0] * d.x + view[4] * d.y) + view[8] * d.z) + view[12] * 0.0f,
+                         ((view[1] * d.x + view[5] * d.y) + view[9] * d.z) + view[13] * 0.0f,
+                         ((view[2] * d.x + view[6] * d.y) + view[10] * d.z) + view[14] * 0.0f);
+                v4 c = color_single(s, eye, g, time, &seed, node_visits, k == 0 ? &d0 : NULL);
+                acc.x = acc.x + c.x;
+                acc.y = acc.y + c.y;
+                acc.z = acc.z + c.z;
+                acc.w = acc.w + c.w;
+            }
+            acc.x = acc.x / 8.0f;
+            acc.y = acc.y / 8.0f;
+            acc.z = acc.z / 8.0f;
+            acc.w = acc.w / 8.0f;
+            size_t o = (size_t)pi;
+            if (rgba) {
+                rgba[4 * o + 0] = orc_det_pow045(orc_linear_to_srgb(acc.x));
+                rgba[4 * o + 1] = orc_det_pow045(orc_linear_to_srgb(acc.y));
+                rgba[4 * o + 2] = orc_det_pow045(orc_linear_to_srgb(acc.z));
+                rgba[4 * o + 3] = acc.w;
+            }
+            if (depth) depth[o] = d0;
+        }
+    }
+    return ORC_OK;
 }
 
 int orc_trace(const orc_state *s, const float view[16], uint32_t W, uint32_t H, float time, float zdist,
@@ -465,6 +726,8 @@ int orc_trace(const orc_state *s, const float view[16], uint32_t W, uint32_t H, 
 {
     uint32_t r1 = rows ? row0 + rows : H;
     if (r1 > H) r1 = H;
+    if (s->p.shading & ORC_SHADE_CINEMATIC)
+        return trace_cinematic(s, view, W, H, time, zdist, rgba, depth, row0, r1, node_visits);
     for (uint32_t y = row0; y < r1; y++) {
         for (uint32_t x = 0; x < W; x++) {
             /* ray_direction, compute.wgsl:4590-4594 */
@@ -480,7 +743,7 @@ int orc_trace(const orc_state *s, const float view[16], uint32_t W, uint32_t H, 
             v3 eye = V(view[12], view[13], view[14]);
             v3 c;
             float dist;
-            trace_ray(s, eye, g, time, &c, &dist, node_visits);
+            trace_ray(s, eye, g, time, &c, &dist, node_visits, NULL);
             /* get_color_single returns vec4(c, 1) on hit and on the first-bounce miss, and
              * vec4(10, 10, 10, 0) on a miss with normal_viz (4510-4514);
              * get_color_supersample: c = (0 + c) / 1, then pow(c, 0.45) per channel. */

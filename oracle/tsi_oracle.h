@@ -71,7 +71,10 @@ typedef struct {
  * no octree darkening.  With ORC_SHADE_GRADIENT (fast_normals = false, 3504) the normal is the
  * voxel's reverse-pass gradient and every cell takes the reverse pass (3780); otherwise it is
  * finite_diff_normal's value for a tape scene, (1, 1, 1) (4143-4184 `default`). */
-enum { ORC_SHADE_NORMALS = 1, ORC_SHADE_GRADIENT = 2 };
+enum { ORC_SHADE_NORMALS = 1, ORC_SHADE_GRADIENT = 2, ORC_SHADE_CINEMATIC = 4 };
+/* cinematic (compute.wgsl:3499): orc_trace renders get_color_supersample's 8-sample path tracer
+ * (4620-4648) over get_color_single's two bounces with sun shadow rays and the sunsky
+ * background (4440-4523, 520-640); every cell takes the reverse pass with ORC_SHADE_GRADIENT. */
 
 typedef struct {
     uint64_t cell_evals;     /* lanes that passed the evaluation guard (3629) */
@@ -100,6 +103,16 @@ float orc_det_logf(float x);
 float orc_det_atan2f(float y, float x);
 float orc_det_powi(float x, int32_t n);
 float orc_det_pow045(float x);
+float orc_det_cosf(float x);
+float orc_det_powf(float x, float y);
+float orc_det_acosf(float x);
+/* cinematic shading pieces (frame.c): frand/frand0to1 (compute.wgsl:2454-2471) advance *seed;
+ * getSampleBiased (4425), sunsky (520) and LinearToSRGB (4600) per channel */
+float orc_frand(uint32_t *seed);
+float orc_frand0to1(uint32_t *seed);
+void orc_sample_biased(const float d[3], float power, uint32_t *seed, float out[3]);
+void orc_sunsky(const float view[3], float spx, float spy, float out[3]);
+float orc_linear_to_srgb(float x);
 float orc_wmin(float a, float b);
 float orc_wmax(float a, float b);
 

@@ -205,7 +205,9 @@ class State:
         p.prune_live_max = prune_live_max
         # shading: 0 = the reference's colouring; TSI_SHADE_NORMALS (1) = normal_viz with
         # finite_diff_normal's (1, 1, 1) for a tape; | TSI_SHADE_GRADIENT (2) = normal_viz with
-        # the voxels' reverse-pass gradients (every cell takes the reverse pass)
+        # the voxels' reverse-pass gradients (every cell takes the reverse pass);
+        # TSI_SHADE_CINEMATIC (4) = the cinematic path tracer (8 samples, 2 bounces, sun and
+        # sky), its normals as above
         p.shading = shading
         self.params = p
         self._ctx = C.c_void_p()

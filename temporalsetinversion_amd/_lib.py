@@ -26,6 +26,7 @@ TSI_FLAG_PRUNE_EXACT_ONLY = 2
 TSI_FLAG_INTERPRET = 4
 TSI_SHADE_NORMALS = 1
 TSI_SHADE_GRADIENT = 2
+TSI_SHADE_CINEMATIC = 4
 TSI_FRAME_RESET = 1
 
 

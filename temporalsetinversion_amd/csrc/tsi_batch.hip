@@ -97,7 +97,7 @@ Layout layout(uint64_t nodes, uint32_t list_cap, size_t px) {
 
 bool params_ok(const tsi_params &p) {
     return p.localsubdiv >= 1 && p.localsubdiv <= kMaxLevels - 4 && std::isfinite(p.delta) &&
-           std::isfinite(p.swept_window) && std::isfinite(p.fast_multiplier);
+           std::isfinite(p.swept_window) && std::isfinite(p.fast_multiplier) && (p.shading & ~kShadeBits) == 0;
 }
 
 }  // namespace
@@ -243,7 +243,7 @@ void session_args(const tsi_batch *b, Session &s, const Layout &L) {
     a.point_sample = s.p.point_sample;
     a.lipschitz = s.p.lipschitz_corrected;
     a.fastenabled = s.p.fastenabled;
-    a.force_deriv = (s.p.shading & TSI_SHADE_NORMALS) && (s.p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
+    a.force_deriv = (s.p.shading & (TSI_SHADE_NORMALS | TSI_SHADE_CINEMATIC)) && (s.p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
     a.prune = 0;
     s.a = a;
     s.rgba = reinterpret_cast<float4 *>(base + L.rgba);
@@ -516,7 +516,7 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
         return b->sessions[reqs[x].session].tape < b->sessions[reqs[y].session].tape;
     });
-    uint32_t njobs = 0, blocks = 0;
+    uint32_t njobs = 0;
     for (uint32_t j = 0; j < n; ++j) {
         const uint32_t i = b->order[j];
         const tsi_frame_req &r = reqs[i];
@@ -547,8 +547,6 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
             J.t.shading = s.p.shading;
             J.t.grad = s.a.grad;
             J.blocks_x = (r.width + 15u) / 16u;
-            J.block0 = blocks;
-            blocks += J.blocks_x * ((r.height + 15u) / 16u);
             b->job_of[i] = int32_t(njobs);
             s.w = r.width;
             s.h = r.height;
@@ -573,10 +571,24 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     HIP_OK(hipEventRecord(b->ev[1], b->stream));
     b->traced = njobs > 0;
     if (njobs) {
+        // cinematic frames after the others, each group with its own launch (the path tracer's
+        // kernel holds more registers; the plain trace keeps its own)
+        TraceJob *jb = b->jobs_h;
+        std::stable_partition(jb, jb + njobs, [](const TraceJob &J) { return !(J.t.shading & TSI_SHADE_CINEMATIC); });
+        uint32_t nplain = 0, bplain = 0, bcin = 0;
+        for (uint32_t j = 0; j < njobs; ++j) {
+            const bool cin = (jb[j].t.shading & TSI_SHADE_CINEMATIC) != 0;
+            uint32_t &acc = cin ? bcin : bplain;
+            jb[j].block0 = acc;
+            acc += jb[j].blocks_x * ((jb[j].t.height + 15u) / 16u);
+            nplain += cin ? 0u : 1u;
+        }
         HIP_OK(hipMemcpyAsync(b->jobs_d, b->jobs_h, njobs * sizeof(TraceJob), hipMemcpyHostToDevice, b->stream));
         if (launch_fill_u32(reinterpret_cast<uint32_t *>(b->visits_d), 0u, 2u * njobs, b->stream)) return TSI_E_HIP;
         HIP_OK(hipEventRecord(b->ev[2], b->stream));
-        if (launch_trace_batch(b->jobs_d, njobs, blocks, b->stream)) return TSI_E_HIP;
+        if (nplain && launch_trace_batch(b->jobs_d, nplain, bplain, false, b->stream)) return TSI_E_HIP;
+        if (nplain < njobs && launch_trace_batch(b->jobs_d + nplain, njobs - nplain, bcin, true, b->stream))
+            return TSI_E_HIP;
         HIP_OK(hipMemcpyAsync(b->visits_h, b->visits_d, njobs * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               b->stream));
     }

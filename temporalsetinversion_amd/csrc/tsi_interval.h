@@ -162,6 +162,41 @@ TSI_DI float det_powi(float x, int n) {
     return n < 0 ? 1.0f / r : r;
 }
 
+// cos: det_sin's reduction with the quadrant advanced by one (cos r = sin(r + pi/2) without
+// rounding x + pi/2).
+TSI_DI float det_cos(float x) {
+    float k = floorf(x * 0.63661977236758134308f + 0.5f);
+    float r = ((x - k * 1.5703125f) - k * 4.837512969970703125e-4f) - k * 7.549790126404332e-08f;
+    float kq = k - 4.0f * floorf(k * 0.25f);
+    int q = int(kq) + 1;
+    float r2 = r * r;
+    float s = r + r * r2 * (-1.66666666666666666667e-1f + r2 * (8.33333333333333333333e-3f +
+              r2 * (-1.98412698412698412698e-4f + r2 * 2.75573192239858906526e-6f)));
+    float c = (1.0f - 0.5f * r2) + r2 * r2 * (4.16666666666666666667e-2f + r2 * (-1.38888888888888888889e-3f +
+              r2 * (2.48015873015873015873e-5f + r2 * -2.75573192239858906526e-7f)));
+    q &= 3;
+    float v = (q & 1) ? c : s;
+    v = (q & 2) ? -v : v;
+    return (x - x == 0.0f) ? v : x - x;
+}
+
+// x^y for the shading formulas: exp(y log x) for x > 0; 0^y is 0 (y > 0), 1 (y = 0) or +inf;
+// x < 0 gives NaN (GLSL/WGSL pow is undefined there); 1^y = 1.
+TSI_DI float det_pow(float x, float y) {
+    float v = det_exp(y * det_log(x));
+    const float z = y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : f_inf());
+    v = x == 0.0f ? z : v;
+    v = x == 1.0f ? 1.0f : v;
+    v = x < 0.0f ? __uint_as_float(0x7fc00000u) : v;
+    return (x != x || y != y) ? x + y : v;
+}
+
+// acos x = atan2(sqrt((1 - x)(1 + x)), x) on [-1, 1]; NaN outside.
+TSI_DI float det_acos(float x) {
+    const float v = det_atan2(sqrtf((1.0f - x) * (1.0f + x)), x);
+    return (x > 1.0f || x < -1.0f) ? __uint_as_float(0x7fc00000u) : v;
+}
+
 // pow(x, 0.45) of get_color_supersample (compute.wgsl:4643-4645) for x in [0, 1].
 TSI_DI float det_pow045(float x) {
     float v = det_exp(0.449999988079071045f * det_log(x));

@@ -82,7 +82,7 @@ struct TraceArgs {
     float4 *rgba;
     float *depth;
     unsigned long long *visits;
-    uint32_t shading;           // TSI_SHADE_* (normal_viz, gradient normals)
+    uint32_t shading;           // TSI_SHADE_* (normal_viz, gradient normals, cinematic)
     const float4 *grad;         // the nodes' dx, dy, dz (gradient normals)
 };
 

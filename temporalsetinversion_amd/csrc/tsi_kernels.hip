@@ -191,16 +191,21 @@ TSI_DI float mylerp(float a, float b, float t) {   // compute.wgsl:4116
     return a * (1.0f - tt) + b * tt;
 }
 
-// compute.wgsl:4189-4403 for the non-cinematic, non-normal_viz colour; the stack lives in LDS.
 // normalize (WGSL), as v / length(v)
 TSI_DI V3 vnormalize(V3 v) {
     const float l = vlength(v);
     return v3(v.x / l, v.y / l, v.z / l);
 }
 
+// trace (compute.wgsl:4189-4403); the stack lives in LDS.  CIN: cinematic colouring (no octree
+// darkening, movement (0.7, 1e-4, 1e-4) fading to base 0.1, 4346-4374) and normal_at_hit in nrm:
+// finite_diff_normal's (1, 1, 1) for a tape scene, or the voxel's gradient with
+// TSI_SHADE_GRADIENT (fast_normals = false, 4333-4342).
+template <bool CIN>
 TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V3 &col, float &dist,
-                      uint32_t &visits, uint32_t shading, const float4 *grad) {
+                      uint32_t &visits, uint32_t shading, const float4 *grad, V3 &nrm) {
     const bool normal_viz = (shading & TSI_SHADE_NORMALS) != 0;
+    nrm = v3(0.0f, 0.0f, 0.0f);
     constexpr uint32_t FAIL = 0xFFFFFFFFu;
     uint32_t sp = 0;
     V3 c = v3(1.0f, 1.0f, 1.0f);
@@ -265,7 +270,7 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
         const bool qown = qnode == region;
         const float hx = (q1.x - q0.x) / 2.0f, hy = (q1.y - q0.y) / 2.0f, hz = (q1.z - q0.z) / 2.0f;
         for (int i = 0; i < 32; ++i) {
-            if (!normal_viz && wmax(p1.x - p0.x, wmax(p1.y - p0.y, p1.z - p0.z)) < 1.0f) {   // 4265
+            if (!CIN && !normal_viz && wmax(p1.x - p0.x, wmax(p1.y - p0.y, p1.z - p0.z)) < 1.0f) {   // 4265
                 c.x = c.x * kDarken; c.y = c.y * kDarken; c.z = c.z * kDarken;
             }
             const float t = ((vlength(v3(mkx * (sdx - ddx), mky * (sdy - ddy), mkz * (sdz - ddz))) + hit.y) + kTrEps) + sms;
@@ -314,18 +319,26 @@ TSI_DI void trace_ray(const KArgs &a, uint32_t *stk, V3 ro, V3 rd, float time, V
                             break;
                         } else {
                             intersected = true;
-                            if (normal_viz) {   // 4333-4377
+                            if (normal_viz || CIN) {
                                 V3 n = v3(1.0f, 1.0f, 1.0f);   // finite_diff_normal of a tape scene
                                 if (shading & TSI_SHADE_GRADIENT) {
                                     const float4 gn = grad[child];
                                     n = v3(gn.x, gn.y, gn.z);
                                 }
-                                const V3 nv = vnormalize(v3(n.x * 1.0f, n.y * 1.0f, n.z * 1.0f));
-                                c = v3(nv.x * 0.5f + 0.5f, nv.y * 0.5f + 0.5f, nv.z * 0.5f + 0.5f);
-                                break;
+                                nrm = n;
+                                if (normal_viz) {   // 4333-4377
+                                    const V3 nv = vnormalize(v3(n.x * 1.0f, n.y * 1.0f, n.z * 1.0f));
+                                    c = v3(nv.x * 0.5f + 0.5f, nv.y * 0.5f + 0.5f, nv.z * 0.5f + 0.5f);
+                                    break;
+                                }
                             }
                             const float tt = (time - cs.y) * 60.0f;
-                            c = v3(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
+                            if (CIN) {
+                                c = v3(c.x - mylerp(1.0f - 0.7f, 1.0f - 0.1f, tt), c.y - mylerp(1.0f - 0.0001f, 1.0f - 0.1f, tt),
+                                       c.z - mylerp(1.0f - 0.0001f, 1.0f - 0.1f, tt));
+                            } else {
+                                c = v3(c.x - mylerp(0.0f, 0.0f, tt), c.y - mylerp(1.0f, 0.0f, tt), c.z - mylerp(1.0f, 0.0f, tt));
+                            }
                             break;
                         }
                     }
@@ -363,9 +376,9 @@ TSI_DI void trace_pixel(const KArgs &a, const TraceArgs &t, uint32_t x, uint32_t
                     ((m[1] * d.x + m[5] * d.y) + m[9] * d.z) + m[13] * 0.0f,
                     ((m[2] * d.x + m[6] * d.y) + m[10] * d.z) + m[14] * 0.0f);
     const V3 eye = v3(m[12], m[13], m[14]);
-    V3 c;
+    V3 c, n;
     float dist;
-    trace_ray(a, stk, eye, g, t.time, c, dist, visits, t.shading, t.grad);
+    trace_ray<false>(a, stk, eye, g, t.time, c, dist, visits, t.shading, t.grad, n);
     const size_t pi = size_t(y) * t.width + x;
     float ca = 1.0f;
     if ((t.shading & TSI_SHADE_NORMALS) && !(dist >= 0.0f)) {
@@ -377,6 +390,202 @@ TSI_DI void trace_pixel(const KArgs &a, const TraceArgs &t, uint32_t x, uint32_t
     t.depth[pi] = dist;
 }
 
+// ---------------------------------------------------------------- cinematic shading
+
+TSI_DI float vdot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+TSI_DI V3 vcross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+constexpr float kPiW = 3.14159274101257324f;   // compute.wgsl:2476 PI as f32
+
+// frand0to1 (compute.wgsl:2466-2471): the LCG step seed * 747796405 + 1, then PCG's RXS-M-XS
+// output permutation, mapped to [0, 1] by / 4294967295.0 (2^32 as f32); frand (2454-2459): [-1, 1].
+TSI_DI float frand01(uint32_t &seed) {
+    seed = seed * 747796405u + 1u;
+    uint32_t w = ((seed >> ((seed >> 28u) + 4u)) ^ seed) * 277803737u;
+    w = (w >> 22u) ^ w;
+    return float(w) / 4294967295.0f;
+}
+TSI_DI float frand(uint32_t &seed) { return frand01(seed) * 2.0f - 1.0f; }
+
+// getSampleBiased (4405-4434): a cosine-power lobe of exponent `power` around d
+TSI_DI V3 sample_biased(V3 d, float power, uint32_t &seed) {
+    const V3 dir = vnormalize(d);
+    const V3 ot = fabsf(dir.x) > fabsf(dir.z) ? v3(-dir.y, dir.x, 0.0f) : v3(0.0f, -dir.z, dir.y);   // ortho
+    const V3 o1 = vnormalize(ot);
+    const V3 o2 = vnormalize(vcross(dir, o1));
+    float rx = frand01(seed);
+    float ry = frand01(seed);
+    rx = rx * 2.0f * kPiW;
+    ry = det_pow(ry, 1.0f / (power + 1.0f));
+    const float om = sqrtf(1.0f - ry * ry);
+    const float cx = det_cos(rx) * om, sy = det_sin(rx) * om;
+    return v3((o1.x * cx + o2.x * sy) + dir.x * ry, (o1.y * cx + o2.y * sy) + dir.y * ry,
+              (o1.z * cx + o2.z * sy) + dir.z * ry);
+}
+
+// fromSpherical (494-518)
+TSI_DI V3 from_spherical(float px, float py) {
+    return v3(det_cos(px) * det_sin(py), det_sin(px) * det_sin(py), det_cos(py));
+}
+
+// sunsky (520-640; Fragmentarium's Sunsky.frag) with turbidity = sunSize = 1 and the sky's sun
+// intensity 1000 (423-427), RayleighPhase (429), totalMie (437), hgPhase (458), SunIntensity
+// (478); pow(x, 2.0) is x * x.  The sun disk it computes is not in its result.
+TSI_DI V3 sunsky(V3 view, float spx, float spy) {
+    const float sadc = det_cos((1.0f * 3.1415927410125732f) / 180.0f);
+    const V3 sd = vnormalize(from_spherical((spx - 0.0f) * 6.28000020980835f, (spy - 0.5f) * 3.140000104904175f));
+    const V3 up = v3(0.0f, 0.0f, 1.0f);
+    const float cvs = vdot(view, sd);
+    const float csu = vdot(sd, up);
+    const float cuv = vdot(up, view);
+    if (sadc == 1.0f) return v3(1.0f, 0.0f, 0.0f);
+    const float sunE = 1000.0f * wmax(0.0f, 1.0f - det_exp(-((1.6110731611233942f - det_acos(csu)) / 1.5f)));
+    const V3 ray = v3(5.176821105123963e-6f, 1.2785348189936485e-5f, 2.8530756026157178e-5f);
+    // totalMie(lambda = (680, 550, 450) nm, K = (0.686, 0.678, 0.666), T = 1) * 0.005
+    const float mc = (0.20000000298023224f * 1.0f) * 9.99999983775159e-18f;
+    const float mk = (0.4339999854564667f * mc) * 3.1415927410125732f;
+    const float tp = 2.0f * 3.1415927410125732f;
+    const float qx = tp / 6.800000278417428e-7f, qy = tp / 5.499999815583578e-7f, qz = tp / 4.4999998749517545e-7f;
+    const V3 mie = v3(((mk * (qx * qx)) * 0.6859999895095825f) * 0.004999999888241291f,
+                      ((mk * (qy * qy)) * 0.6779999732971191f) * 0.004999999888241291f,
+                      ((mk * (qz * qz)) * 0.6660000085830688f) * 0.004999999888241291f);
+    const float zen = wmax(0.0f, cuv);
+    const float rol = 8400.0f / zen, mol = 1250.0f / zen;
+    const V3 fex = v3(det_exp(-((ray.x * rol) + (mie.x * mol))), det_exp(-((ray.y * rol) + (mie.y * mol))),
+                      det_exp(-((ray.z * rol) + (mie.z * mol))));
+    const float rp = (3.0f / (16.0f * 3.1415927410125732f)) * (1.0f + cvs * cvs);
+    const float g = 0.800000011920929f;
+    const float hp = (1.0f / (4.0f * 3.1415927410125732f)) * ((1.0f - g * g) / det_pow((1.0f - (2.0f * g) * cvs) + g * g, 1.5f));
+    const V3 lx = v3(ray.x * rp + mie.x * hp, ray.y * rp + mie.y * hp, ray.z * rp + mie.z * hp);
+    const V3 tl = v3(ray.x + mie.x, ray.y + mie.y, ray.z + mie.z);
+    const V3 se = v3(sunE * (lx.x / tl.x), sunE * (lx.y / tl.y), sunE * (lx.z / tl.z));
+    const float f = wmin(wmax(det_pow(1.0f - vdot(up, sd), 5.0f), 0.0f), 1.0f);
+    const V3 mx = v3(1.0f * (1.0f - f) + det_pow(se.x * fex.x, 0.5f) * f, 1.0f * (1.0f - f) + det_pow(se.y * fex.y, 0.5f) * f,
+                     1.0f * (1.0f - f) + det_pow(se.z * fex.z, 0.5f) * f);
+    return v3(((se.x * (1.0f - fex.x)) * mx.x) * 0.009999999776482582f, ((se.y * (1.0f - fex.y)) * mx.y) * 0.009999999776482582f,
+              ((se.z * (1.0f - fex.z)) * mx.z) * 0.009999999776482582f);
+}
+
+// LinearToSRGB (4600-4607) per channel, after clamping to [0, 1]
+TSI_DI float linear_to_srgb(float x) {
+    const float r = wmin(wmax(x, 0.0f), 1.0f);
+    const float a = det_pow(r, 0.416666666666666667f) * 1.055f - 0.055f;
+    const float b = r * 12.92f;
+    const float t = r < 0.0031308f ? 1.0f : 0.0f;
+    return a * (1.0f - t) + b * t;
+}
+
+// One pixel of rt_main with cinematic = true (compute.wgsl:4651-4688, 4620-4648, 4440-4523):
+// seed3 from the pixel index and the time, 8 samples jittered by frand() * 0.5, each
+// get_color_single's path of two bounces, each bounce a sun shadow ray (traced only when its
+// result is read: sunlight > 0, 4501), a miss after the first bounce sees the sky; the mean goes
+// through LinearToSRGB and the 0.45 gamma, alpha is the mean alpha.  The lighting is the
+// scene-independent one (sun at (-0.1, 0.7), intensity 3, specular 1, albedo 1, sun size 0.01).
+// Every ray of the pixel goes through the one trace_ray call below: a lane's rays follow each
+// other in a small state machine (sample, bounce, shadow), so a wavefront's lanes trace together
+// wherever their paths are.  depth: the first sample's first ray.
+TSI_DI void trace_pixel_cin(const KArgs &a, const TraceArgs &t, uint32_t x, uint32_t y, uint32_t *stk, uint32_t &visits) {
+    constexpr float spx = -0.1f, spy = 0.7f, sun_i = 3.0f, specular_i = 1.0f, albedo = 1.0f, sunsize = 0.01f,
+                    bgmul = 1.0f;
+    const uint32_t pidx = y * t.width + x;
+    uint32_t seed = pidx + uint32_t(12345.0f * (det_sin(t.time * 12345.0f) * 0.5f + 0.5f));
+    const float *m = t.view;
+    const V3 eye = v3(m[12], m[13], m[14]);
+    const V3 sdn = vnormalize(from_spherical((spx - 0.0f) * 6.28f, (spy - 0.5f) * 3.14f));
+    const V3 sundir = v3(sdn.x, sdn.z, sdn.y);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, aw = 0.0f;
+    float depth0 = -1.0f;
+    V3 frm = eye, dir = eye, lum = eye, direct = eye, old_rd = eye, nh = eye, ssd = eye;
+    float sunlight = 0.0f;
+    uint32_t sample = 0, bounce = 0;
+    bool shadow = false;
+    auto begin_sample = [&]() {
+        const float jx = frand(seed);
+        const float jy = frand(seed);
+        const float px = float(x) + (jx * 1.0f) * 0.5f, py = float(y) + (jy * 1.0f) * 0.5f;
+        const float xyx = (px - float(t.width) / 2.0f) + 0.5f;
+        const float xyy = (py - float(t.height) / 2.0f) + 0.5f;
+        V3 d = v3(xyx, -xyy, -t.zdist);
+        const float len = vlength(d);
+        d = v3(d.x / len, d.y / len, d.z / len);
+        dir = v3(((m[0] * d.x + m[4] * d.y) + m[8] * d.z) + m[12] * 0.0f,
+                 ((m[1] * d.x + m[5] * d.y) + m[9] * d.z) + m[13] * 0.0f,
+                 ((m[2] * d.x + m[6] * d.y) + m[10] * d.z) + m[14] * 0.0f);
+        frm = eye;
+        lum = v3(1.0f, 1.0f, 1.0f);
+        direct = v3(0.0f, 0.0f, 0.0f);
+        bounce = 0;
+        shadow = false;
+    };
+    begin_sample();
+    for (uint32_t it = 0; it < 32u; ++it) {   // 8 samples x 2 bounces x (ray, shadow ray)
+        V3 c, n;
+        float d;
+        trace_ray<true>(a, stk, frm, shadow ? ssd : dir, t.time, c, d, visits, t.shading, t.grad, n);
+        bool end_bounce = false, end_sample = false;
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f, rw = 0.0f;
+        if (shadow) {
+            shadow = false;
+            if (d < 0.0f) {   // not in shadow
+                direct = v3(direct.x + (lum.x * sunlight) * sun_i, direct.y + (lum.y * sunlight) * sun_i,
+                            direct.z + (lum.z * sunlight) * sun_i);
+                const V3 half = vnormalize(v3(-old_rd.x + sundir.x, -old_rd.y + sundir.y, -old_rd.z + sundir.z));
+                const float spec = (100.0f * det_pow(wmax(vdot(nh, half), 0.0f), 30.0f)) * specular_i;
+                direct = v3(direct.x + lum.x * spec, direct.y + lum.y * spec, direct.z + lum.z * spec);
+            }
+            end_bounce = true;
+        } else {
+            if (sample == 0u && bounce == 0u) depth0 = d;
+            if (d >= 0.0f) {
+                const V3 hp = v3(frm.x + dir.x * d, frm.y + dir.y * d, frm.z + dir.z * d);
+                old_rd = dir;
+                dir = sample_biased(n, 3.0f, seed);
+                lum = v3((lum.x * c.x) * albedo, (lum.y * c.y) * albedo, (lum.z * c.z) * albedo);
+                frm = v3(hp.x + n.x * 0.05f, hp.y + n.y * 0.05f, hp.z + n.z * 0.05f);
+                const float j0 = frand(seed);
+                const float j1 = frand(seed);
+                const float j2 = frand(seed);
+                ssd = vnormalize(v3(sundir.x + j0 * sunsize, sundir.y + j1 * sunsize, sundir.z + j2 * sunsize));
+                sunlight = vdot(n, ssd);
+                nh = n;
+                if (sunlight > 0.0f) shadow = true;
+                else end_bounce = true;
+            } else if (bounce == 0u) {   // see_background = false (3506)
+                rx = 10.0f; ry = 10.0f; rz = 10.0f; rw = 0.0f;
+                end_sample = true;
+            } else {
+                const V3 bg = sunsky(v3(dir.x, dir.z, dir.y), spx, spy);
+                rx = (direct.x + lum.x * bg.x) * bgmul;
+                ry = (direct.y + lum.y * bg.y) * bgmul;
+                rz = (direct.z + lum.z * bg.z) * bgmul;
+                rw = 1.0f * bgmul;
+                end_sample = true;
+            }
+        }
+        if (end_bounce && ++bounce == 2u) {
+            rx = direct.x; ry = direct.y; rz = direct.z; rw = 1.0f;
+            end_sample = true;
+        }
+        if (end_sample) {
+            ax = ax + rx; ay = ay + ry; az = az + rz; aw = aw + rw;
+            if (++sample == 8u) break;
+            begin_sample();
+        }
+    }
+    ax = ax / 8.0f; ay = ay / 8.0f; az = az / 8.0f; aw = aw / 8.0f;
+    const size_t pi = size_t(pidx);
+    t.rgba[pi] = make_float4(det_pow045(linear_to_srgb(ax)), det_pow045(linear_to_srgb(ay)),
+                             det_pow045(linear_to_srgb(az)), aw);
+    t.depth[pi] = depth0;
+}
+
+template <bool CIN>
+TSI_DI void shade_pixel(const KArgs &a, const TraceArgs &t, uint32_t x, uint32_t y, uint32_t *stk, uint32_t &visits) {
+    if (CIN) trace_pixel_cin(a, t, x, y, stk, visits);
+    else trace_pixel(a, t, x, y, stk, visits);
+}
+
+template <bool CIN>
 __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     __shared__ uint32_t stack_lds[16 * 256];
     // a 16x16 block of pixels; each wavefront takes one 8x8 quadrant (square tiles keep the
@@ -385,7 +594,7 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
     const uint32_t x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
     const uint32_t y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
     uint32_t visits = 0;
-    if (x < t.width && y < t.height) trace_pixel(a, t, x, y, stack_lds + threadIdx.x, visits);
+    if (x < t.width && y < t.height) shade_pixel<CIN>(a, t, x, y, stack_lds + threadIdx.x, visits);
     uint64_t v = wave_sum64(visits);
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
 }
@@ -395,6 +604,7 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
 // and its wavefronts take the 8x8 quadrants, as k_trace's do (a persistent grid taking tiles
 // from a counter measured slower: a frame's slowest tiles then ran one after another in a few
 // wavefronts, where the hardware dispatcher starts each block on the first free slot).
+template <bool CIN>
 __global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint32_t njobs) {
     __shared__ uint32_t stack_lds[16 * 256];
     const uint32_t blk = blockIdx.x;
@@ -425,7 +635,7 @@ __global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint3
     const uint32_t x = (local % J.blocks_x) * 16u + (w & 1u) * 8u + (l & 7u);
     const uint32_t y = (local / J.blocks_x) * 16u + (w >> 1) * 8u + (l >> 3);
     uint32_t visits = 0;
-    if (x < t.width && y < t.height) trace_pixel(a, t, x, y, stack_lds + threadIdx.x, visits);
+    if (x < t.width && y < t.height) shade_pixel<CIN>(a, t, x, y, stack_lds + threadIdx.x, visits);
     const uint64_t v = wave_sum64(visits);
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
 }
@@ -519,8 +729,9 @@ int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_batch, dim3(total_blocks), dim3(256), 0, s, jobs, njobs);
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, bool cinematic, hipStream_t s) {
+    if (cinematic) hipLaunchKernelGGL(k_trace_batch<true>, dim3(total_blocks), dim3(256), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_trace_batch<false>, dim3(total_blocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -536,7 +747,8 @@ int launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s) {
 
 int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s) {
     dim3 grid((t.width + 15u) / 16u, (t.height + 15u) / 16u);
-    hipLaunchKernelGGL(k_trace, grid, dim3(256), 0, s, a, t);
+    if (t.shading & TSI_SHADE_CINEMATIC) hipLaunchKernelGGL(k_trace<true>, grid, dim3(256), 0, s, a, t);
+    else hipLaunchKernelGGL(k_trace<false>, grid, dim3(256), 0, s, a, t);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -9,6 +9,9 @@
 
 namespace tsi {
 
+// every TSI_SHADE_* bit; tsi_params.shading with any other bit is rejected
+constexpr uint32_t kShadeBits = TSI_SHADE_NORMALS | TSI_SHADE_GRADIENT | TSI_SHADE_CINEMATIC;
+
 struct FrameLaunch {
     int levels;
     float time;
@@ -34,7 +37,8 @@ struct TraceJob {
     uint32_t blocks_x;  // ceil(width / 16): 16x16-pixel blocks per row
     uint32_t block0;    // the frame's first block in the launch
 };
-int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, hipStream_t s);
+// (cinematic: every job's session shades with TSI_SHADE_CINEMATIC; no job mixes the two)
+int launch_trace_batch(const TraceJob *jobs, uint32_t njobs, uint32_t total_blocks, bool cinematic, hipStream_t s);
 // The canonical node records of an octree (tsi_read_nodes): every node reachable from the root
 // through nodes with children, keyed by (depth, i, j, k).  Synchronous device reads.
 int read_octree(const float4 *st_dev, const float4 *grad_dev, const Counters *cnt_dev, uint64_t nodes, tsi_node *out,

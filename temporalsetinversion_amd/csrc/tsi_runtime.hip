@@ -180,7 +180,7 @@ KArgs make_args(const tsi_ctx *c) {
     a.point_sample = c->p.point_sample;
     a.lipschitz = c->p.lipschitz_corrected;
     a.fastenabled = c->p.fastenabled;
-    a.force_deriv = (c->p.shading & TSI_SHADE_NORMALS) && (c->p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
+    a.force_deriv = (c->p.shading & (TSI_SHADE_NORMALS | TSI_SHADE_CINEMATIC)) && (c->p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
     a.prune = c->prune ? 1 : 0;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) a.prune_rec[i][j] = c->prune_rec[i][j];
@@ -447,6 +447,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
     if (!params || !out) return TSI_E_ARG;
     *out = nullptr;
     if (params->localsubdiv < 1 || params->localsubdiv > kMaxLevels - 4) return TSI_E_ARG;
+    if (params->shading & ~kShadeBits) return TSI_E_ARG;
     tsi_ctx *c = new (std::nothrow) tsi_ctx();
     if (!c) return TSI_E_OOM;
     c->p = *params;
@@ -763,7 +764,7 @@ int tsi_trace_async(tsi_ctx *c) {
     }
     KArgs a = make_args(c);
     // snapshot of the nodes in use, in stream order after the last set inversion
-    const bool gradient_normals = (c->p.shading & TSI_SHADE_NORMALS) && (c->p.shading & TSI_SHADE_GRADIENT);
+    const bool gradient_normals = (c->p.shading & (TSI_SHADE_NORMALS | TSI_SHADE_CINEMATIC)) && (c->p.shading & TSI_SHADE_GRADIENT);
     if (gradient_normals && !ts.grad && (rc = dmalloc(&ts.grad, c->nodes))) return rc;
     if (launch_snapshot(a, ts.st, ts.blo, ts.bhi, gradient_normals ? ts.grad : nullptr, c->nodes, c->grid_misc,
                         c->stream) != 0)

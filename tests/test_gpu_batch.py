@@ -25,9 +25,13 @@ SESSIONS = [
     (0, 4, dict(point_sample=False, swept_volume=True, swept_window=0.5), (32, 32)),
     (1, 6, dict(swept_volume=False, fast_multiplier=2.0), (0, 0)),
     (3, 4, dict(delta=0.02, lipschitz_corrected=True), (24, 16)),
+    # cinematic path tracer (TSI_SHADE_CINEMATIC), then with gradient normals as well: traced by
+    # their own launch beside the plain trace of the others
+    (2, 5, dict(shading=4), (40, 32)),
+    (0, 5, dict(shading=6), (32, 24)),
 ]
 # which sessions take part in each submit, and the time of each session's k-th frame
-SUBMITS = [[0, 1, 2, 3, 4, 5, 6], [0, 2, 3, 5], [1, 2, 4, 6], [0, 1, 2, 3, 4, 5, 6]]
+SUBMITS = [[0, 1, 2, 3, 4, 5, 6, 7, 8], [0, 2, 3, 5, 7], [1, 2, 4, 6, 8], [0, 1, 2, 3, 4, 5, 6, 7, 8]]
 
 
 def frame_time(sid, k):

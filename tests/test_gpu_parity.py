@@ -265,6 +265,45 @@ def test_frames_outside_the_specialised_domain():
         assert not st.tape_info()["specialized"]
 
 
+@pytest.mark.parametrize("scene,shading,L,size,kw", [
+    ("spheres", 4, 6, (64, 48), {}),                              # cinematic, normals (1, 1, 1)
+    ("capsules", 6, 6, (48, 36), dict(lipschitz_corrected=True)),  # + gradient normals
+    ("csg", 5, 5, (40, 30), dict(swept_volume=True)),              # + normal_viz
+])
+def test_cinematic_matches_oracle(scene, shading, L, size, kw):
+    """The cinematic path tracer (compute.wgsl:3499, 4440-4523, 4620-4648): colour, depth and
+    node visits bit-identical to the oracle over three frames (jitter, bounces, sun shadow rays,
+    sky; the seed follows the time)."""
+    import oracle
+
+    tape = tape_of(scene)
+    W, H = size
+    view = T.look_at_inverse((2.2, 1.7, 2.6))
+    fr = oracle.Frame(tape, localsubdiv=L, voxel_capacity=1 << 20, shading=shading, **kw)
+    st = T.State(tape, width=W, height=H, localsubdiv=L, voxel_capacity=1 << 20, shading=shading, **kw)
+    try:
+        st.reset()
+        for k, t in enumerate([0.0, 1 / 60, 0.25]):
+            t = float(np.float32(t))
+            st.update(t, view)
+            gs = st.set_inversion().stats
+            os_ = fr.set_inversion(t)
+            assert gs["rev_evals"] == os_.rev_evals
+            grgba, gdepth = st.trace()
+            _, gvisits = st.last_trace()
+            orgba, odepth, ovisits = fr.trace(np.array(st.uniforms.view[:], np.float32), W, H, t, T.camera_zdist(H))
+            print(f"{scene} shading={shading} frame={k} visits={gvisits}", flush=True)
+            assert_same_nodes(st.read_nodes(), fr.export(), f"frame {k}")
+            assert np.array_equal(grgba.view(np.uint32), orgba.view(np.uint32)), f"colour differs in frame {k}"
+            assert np.array_equal(gdepth.view(np.uint32), odepth.view(np.uint32)), f"depth differs in frame {k}"
+            assert gvisits == ovisits
+            st.enqueue_root()
+            fr.enqueue_root()
+    finally:
+        st.close()
+        fr.close()
+
+
 @pytest.mark.parametrize("shading", [1, 3])   # TSI_SHADE_NORMALS, | TSI_SHADE_GRADIENT
 def test_normal_viz_matches_oracle(shading):
     """normal_viz (compute.wgsl:3503-3504): octree and images bit-identical to the oracle, with
