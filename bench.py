@@ -130,28 +130,29 @@ def summarize_profile(args) -> None:
                  "--pmc-fetch <FETCH_SIZE counter_collection.csv> --pmc-write <WRITE_SIZE ...> --copy-fetch <...> "
                  "--copy-write <...> --pmc-sq <SQ pass csv files>")
     lines += ["#", f"# {frames} frames ({skip} warm-up, {frames - skip} timed); one tape-evaluation launch per octree "
-              f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, one snapshot + "
+              f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, after one more "
+              "launch at frame start (the speculative pass of the", "# shallow levels; not in frame 0), one snapshot + "
               "trace per frame on a trace stream.", ""]
     lines.append(f"{'kernel':<40} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
     for name, (n, t) in sorted(per.items(), key=lambda x: -x[1][1]):
         lines.append(f"{name[:40]:<40} {n:>10} {t / 1e3:>10.3f} {t / n:>9.2f} {100 * t / total:>6.1f}%")
-    # per octree level (the k-th evaluation launch of a frame runs level k)
-    lvl, per_level = -1, {}
+    # per octree level: the k-th evaluation launch of a frame runs level k, after the frame's
+    # speculative pass when the frame has one launch more than levels
+    evals = {}
     for r, f in zip(rows, frame_of):
-        name = _kernel(r["Kernel_Name"])
-        if name == "tsi::k_frame_begin":
-            lvl = 0
-        elif name == "tsi_tape_eval_level" and f >= skip:
-            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
-            per_level.setdefault(lvl, []).append(d)
-            lvl += 1
-        elif name == "tsi_tape_eval_level":
-            lvl += 1
+        if _kernel(r["Kernel_Name"]) == "tsi_tape_eval_level" and f >= skip:
+            evals.setdefault(f, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    per_level = {}
+    for f, ds in evals.items():
+        labels = (["spec"] if len(ds) == L + 1 else []) + list(range(L))
+        for lab, d in zip(labels, ds):
+            per_level.setdefault(lab, []).append(d)
     if per_level:
-        lines += ["", "tsi_tape_eval_level per octree level, timed frames (us): level  mean  min  max"]
-        for d in sorted(per_level):
+        lines += ["", "tsi_tape_eval_level per octree level, timed frames (us): level  mean  min  max",
+                  "(spec: the frame-start speculative pass over the previous frame's shallow frontier)"]
+        for d in sorted(per_level, key=lambda x: -1 if x == "spec" else x):
             v = per_level[d]
-            lines.append(f"  level {d:>2} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
+            lines.append(f"  level {d:>4} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
     lines.append("")
     lines.append("timed frames only (the figures bench.py reads):")
     for name in ("tsi_tape_eval_level", "tsi::k_trace"):

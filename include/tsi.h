@@ -104,7 +104,16 @@ enum {
     /* Evaluate tapes with the interpreter (programs in constant memory) instead of compiling
      * each uploaded tape into straight-line evaluation kernels (which prune by regions of
      * 16 records, see prune_live_max). */
-    TSI_FLAG_INTERPRET = 4u
+    TSI_FLAG_INTERPRET = 4u,
+    /* Do not evaluate the shallow levels speculatively.  By default the compiled engine, when
+     * the tape is not pruned, evaluates at each frame's start, in one launch, the children of
+     * the previous frame's frontier parents of the first levels (up to 7, $TSI_SPEC_LEVELS
+     * sets another count, 0 none), where each level otherwise runs one wavefront's dependent
+     * chain through the tape on an idle device; a level then takes a cell's results from
+     * that launch when they were computed from the cell's inputs of this frame (same parent
+     * fast flag and t-window) and evaluates the others.  Results and statistics are the same
+     * either way; the cells whose parents are not enqueued this frame are extra work. */
+    TSI_FLAG_NO_SPECULATE = 8u
 };
 
 /* Per-frame statistics.  fwd_ops/rev_ops count algorithmic f32 interval operations. */
@@ -125,6 +134,8 @@ typedef struct {
     double eval_kernel_ms;        /* the evaluation launches alone: device time between events
                                      recorded right before and after each level's launches
                                      (eval_ms also holds the gaps between levels) */
+    uint64_t spec_cells;          /* of cell_evals, cells whose results came from the frame's
+                                     speculative pass (TSI_FLAG_NO_SPECULATE) */
 } tsi_stats;
 
 /* Canonical node record (see tsi_read_nodes). */

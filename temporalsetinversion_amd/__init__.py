@@ -172,7 +172,7 @@ class State:
                  point_sample: bool = True, swept_volume: bool = False, lipschitz_corrected: bool = False,
                  fastenabled: bool = True, fast_multiplier: float = 1.0, swept_window: float = 1.0 / 60.0,
                  voxel_capacity: int = 1 << 24, queue_capacity: int = 0, prune=True, prune_capacity: int = 0,
-                 engine: str = "compiled", prune_live_max: int = 0, shading: int = 0):
+                 engine: str = "compiled", prune_live_max: int = 0, shading: int = 0, speculate: bool = True):
         if isinstance(tape, IR):
             tape = tape.gpu_instructions
         elif isinstance(tape, (str, bytes)):
@@ -201,6 +201,10 @@ class State:
             raise ValueError(f"engine must be 'compiled' or 'interpreter', not {engine!r}")
         if engine == "interpreter":
             p.flags |= _lib.TSI_FLAG_INTERPRET
+        # speculate: the compiled, unpruned engine evaluates the shallow levels of each frame in
+        # one launch at its start from the previous frame's frontier (TSI_FLAG_NO_SPECULATE)
+        if not speculate:
+            p.flags |= _lib.TSI_FLAG_NO_SPECULATE
         p.prune_capacity = prune_capacity
         p.prune_live_max = prune_live_max
         # shading: 0 = the reference's colouring; TSI_SHADE_NORMALS (1) = normal_viz with

@@ -24,6 +24,7 @@ ERRORS = {
 TSI_FLAG_NO_PRUNE = 1
 TSI_FLAG_PRUNE_EXACT_ONLY = 2
 TSI_FLAG_INTERPRET = 4
+TSI_FLAG_NO_SPECULATE = 8
 TSI_SHADE_NORMALS = 1
 TSI_SHADE_GRADIENT = 2
 TSI_SHADE_CINEMATIC = 4
@@ -62,7 +63,7 @@ class Stats(C.Structure):
                 ("rev_evals", C.c_uint64), ("fwd_ops", C.c_uint64), ("rev_ops", C.c_uint64),
                 ("parents", C.c_uint64), ("deallocs", C.c_uint64), ("enqueued", C.c_uint64),
                 ("allocated_nodes", C.c_uint64), ("eval_launches", C.c_uint64), ("pruned_instr", C.c_uint64),
-                ("prune_launches", C.c_uint64), ("eval_kernel_ms", C.c_double)]
+                ("prune_launches", C.c_uint64), ("eval_kernel_ms", C.c_double), ("spec_cells", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -636,6 +636,7 @@ int tsi_batch_wait(tsi_batch *b, tsi_frame_result *results, double *set_ms, doub
         r.stats.fwd_ops = h.stats[kStatFwdOps];
         r.stats.rev_ops = h.stats[kStatRevOps];
         r.stats.pruned_instr = h.stats[kStatPruned];
+        r.stats.spec_cells = h.stats[kStatSpec];
         r.stats.parents = h.stats[kStatParents];
         r.stats.deallocs = h.stats[kStatDeallocs];
         r.stats.enqueued = h.stats[kStatEnqueued];

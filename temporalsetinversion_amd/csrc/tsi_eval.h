@@ -56,7 +56,7 @@ TSI_DI uint64_t dealloc_step(const KArgs &a, int step, uint32_t wave, uint32_t n
 // Per-lane tallies of the cells a wavefront processed, summed over the wavefront and added to
 // the counters once (flush_tally).
 struct LevelTally {
-    uint64_t cells = 0, rev = 0, par = 0, enq = 0, rev_ops = 0, fwd_ops = 0, pruned = 0, freed = 0;
+    uint64_t cells = 0, rev = 0, par = 0, enq = 0, rev_ops = 0, fwd_ops = 0, pruned = 0, freed = 0, spec = 0;
 };
 
 // Frontier entries of one list at a level (a list that overflowed, flagged by wave_append,
@@ -67,15 +67,31 @@ TSI_DI uint32_t list_entries(const KA &a, int level, int which) {
     return min(a.cnt->list_count[level][which], a.list_cap);
 }
 
+// Appends v for every lane with `want` to out[0 .. cap) (one atomic per wavefront); the count
+// keeps growing past cap, the entries beyond it are dropped.
+TSI_DI void append_capped(bool want, uint32_t v, uint32_t *out, uint32_t *count, uint32_t cap) {
+    const uint64_t m = ballot(want);
+    if (!m) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, popc64(m));
+    base = bcast(base, 0);
+    const uint32_t at = base + rank_in(m);
+    if (want && at < cap) out[at] = v;
+}
+
 // Group g (entries 8g .. 8g + 7) of one frontier list at one level: 8 parents x 8 child octants
 // on the wavefront's 64 lanes.  count = list_entries(a, level, REV); stk = the wavefront's
 // saved-operand stack (uniform base; lane l uses column l of each row).
+// spec (reverse list only): the speculative pass of the frame's start: the entries are the
+// candidate parents (a.spec_in), every child of a parent that has children is evaluated as the
+// octree stands, and only its SpecCell record and the parent's tag are written.
 template <bool REV, class Engine, class KA>
 TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t count, float2 *stk, Engine &E,
-                       Slots &Rlo, Slots &Rhi, LevelTally &tl) {
+                       Slots &Rlo, Slots &Rhi, LevelTally &tl, bool spec = false) {
     const int which = REV ? 1 : 0;
+    const bool sp = REV && spec;
     Counters *cnt = a.cnt;
-    const uint32_t *list = a.lists[level & 1][which];
+    const uint32_t *list = sp ? a.spec_in : a.lists[level & 1][which];
     uint32_t *next_f = a.lists[(level + 1) & 1][0];
     uint32_t *next_r = a.lists[(level + 1) & 1][1];
     const uint32_t lane = lane_id();
@@ -100,7 +116,7 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
         // ---- allocate child blocks for new parents: one atomic per wavefront ----
         uint32_t blk = (pmeta & kChildMask) - 1u;
         {
-            const uint64_t req = ballot(isnew && o == 0);
+            const uint64_t req = sp ? 0u : ballot(isnew && o == 0);
             if (req) {
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(&cnt->alloc_tickets, popc64(req));
@@ -114,13 +130,13 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
                 }
             }
         }
-        bool ok = active;
-        if (isnew && blk >= a.nblocks) {
+        bool ok = active && !(sp && isnew);
+        if (isnew && ok && blk >= a.nblocks) {
             ok = false;
             if (o == 0) atomicOr(&cnt->frame_error, kErrPool);
         }
         if (isnew && ok && o == 0) a.st[P].w = __uint_as_float((pmeta & kFastBit) | (blk + 1u));
-        tl.par += (active && o == 0) ? 1 : 0;
+        if (!sp) tl.par += (active && o == 0) ? 1 : 0;
 
         const uint32_t C = ok ? blk * 8u + o : 0u;
         float4 cst = make_float4(1.0f, 0.0f, 0.0f, 0.0f);   // mallocV (compute.wgsl:3437-3449)
@@ -134,7 +150,8 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
         Iv cx = I(plo.x + dx * float(lx), plo.x + dx * float(1u + lx));
         Iv cy = I(plo.y + dy * float(ly), plo.y + dy * float(1u + ly));
         Iv cz = I(plo.z + dz * float(lz), plo.z + dz * float(1u + lz));
-        if (ok) {
+        // (a node keeps its box while it is allocated: only new nodes write it)
+        if (ok && isnew) {
             a.blo[C] = make_float4(cx.lo, cy.lo, cz.lo, 0.0f);
             a.bhi[C] = make_float4(cx.hi, cy.hi, cz.hi, 0.0f);
         }
@@ -188,24 +205,73 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
         const bool need_choices = prune && ballot(guard && kids_ok && leaf_w > eps) != 0;
         E.select(a, which, g, pruned_ok, prec_in, rec_idx, need_choices);
 
+        // ---- this cell's speculative results: the frame-start pass evaluated it (its parent
+        //      carries this frame's tag) from the same inputs ----
+        bool have = false;
+        const SpecCell *sr = nullptr;
+        if (!sp && level < int(a.spec_levels) && ok && !isnew && __float_as_uint(phi.w) == a.spec_tag) {
+            const uint32_t j = __float_as_uint(plo.w);
+            if (j < a.spec_cap) {
+                sr = a.spec_res + size_t(j) * 8u + o;
+                const uint32_t fl = sr->flags;
+                have = guard && sr->parent == P && (fl & kSpecEval) != 0 && ((fl & kSpecPfast) != 0) == pfast &&
+                       __float_as_uint(sr->targ_lo) == __float_as_uint(targ.lo) &&
+                       __float_as_uint(sr->targ_hi) == __float_as_uint(targ.hi) && (!REV || !td || (fl & kSpecRev) != 0);
+            }
+        }
+
         // ---- tape evaluation (wave-uniform) ----
         Iv field = I_FROM(0.0f);
-        const uint64_t need_eval = ballot(guard);
+        uint32_t fcv = 0;
+        const uint64_t need_eval = ballot(guard && !have);
         if (need_eval) {
-            const uint32_t fc = E.template forward<REV>(a, cx, cy, cz, targ, stk, field, Rlo, Rhi);
-            if (guard) {
-                tl.fwd_ops += fc;
+            fcv = E.template forward<REV>(a, cx, cy, cz, targ, stk, field, Rlo, Rhi);
+            if (guard && !have && !sp) {
+                tl.fwd_ops += fcv;
                 tl.pruned += a.nfwd - E.records();
             }
+        }
+        if (have) {
+            field = I(sr->field_lo, sr->field_hi);
+            tl.fwd_ops += sr->fc;
         }
         Iv df = I_FROM(0.0f), df_dt = I_FROM(0.0f);
         float4 gradv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float terr = cst.z;
         if (REV) {
-            const uint64_t need_rev = ballot(guard && td);
-            if (need_rev) {
-                Iv sens[4];
-                const uint32_t ops = E.reverse(a, stk, sens, Rlo, Rhi);
+            const uint64_t need_rev = ballot(guard && td && !have);
+            Iv sens[4] = {I_FROM(0.0f), I_FROM(0.0f), I_FROM(0.0f), I_FROM(0.0f)};
+            uint32_t ops = 0;
+            if (need_rev) ops = E.reverse(a, stk, sens, Rlo, Rhi);
+            if (have && td) {
+                for (int k = 0; k < 4; ++k) sens[k] = I(sr->sens[2 * k], sr->sens[2 * k + 1]);
+                ops = sr->ops;
+            }
+            if (sp) {
+                // the record of the speculative pass, and the parent's tag for this frame
+                if (ok) {
+                    SpecCell r;
+                    r.field_lo = field.lo;
+                    r.field_hi = field.hi;
+                    for (int k = 0; k < 4; ++k) {
+                        r.sens[2 * k] = sens[k].lo;
+                        r.sens[2 * k + 1] = sens[k].hi;
+                    }
+                    r.targ_lo = targ.lo;
+                    r.targ_hi = targ.hi;
+                    r.fc = fcv;
+                    r.ops = ops;
+                    r.flags = (guard ? kSpecEval : 0u) | (guard && td ? kSpecRev : 0u) | (pfast ? kSpecPfast : 0u);
+                    r.parent = P;
+                    a.spec_res[size_t(g * 8u + ps) * 8u + o] = r;
+                    if (o == 0) {
+                        reinterpret_cast<float *>(&a.blo[P])[3] = __uint_as_float(g * 8u + ps);
+                        reinterpret_cast<float *>(&a.bhi[P])[3] = __uint_as_float(a.spec_tag);
+                    }
+                }
+                return;
+            }
+            {
                 if (guard && td) {
                     tl.rev_ops += ops;
                     const Iv s0 = sens[0], s1 = sens[1], s2 = sens[2];
@@ -287,7 +353,10 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
             if (!valid) rec = nullptr;
             if (ballot(rec != nullptr) != 0) E.liveness(a, rec);
         }
+        // this level's parents are the next frame's speculative candidates
+        if (level < int(a.spec_levels)) append_capped(active && o == 0, P, a.spec_out, a.spec_out_count, a.spec_cap);
         tl.cells += guard ? 1 : 0;
+        tl.spec += have ? 1 : 0;
         tl.rev += (guard && td) ? 1 : 0;
         tl.enq += enq ? 1 : 0;
     }
@@ -297,7 +366,7 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
 TSI_DI void flush_tally(Counters *cnt, const LevelTally &t) {
     const uint64_t n_cells = wave_sum64(t.cells), n_rev = wave_sum64(t.rev), n_par = wave_sum64(t.par),
                    n_enq = wave_sum64(t.enq), rev_ops = wave_sum64(t.rev_ops), fwd_ops = wave_sum64(t.fwd_ops),
-                   n_pruned = wave_sum64(t.pruned), n_freed = wave_sum64(t.freed);
+                   n_pruned = wave_sum64(t.pruned), n_freed = wave_sum64(t.freed), n_spec = wave_sum64(t.spec);
     if (lane_id() == 0) {
         if (n_cells) atomicAdd((unsigned long long *)&cnt->stats[kStatCells], (unsigned long long)n_cells);
         if (n_rev) atomicAdd((unsigned long long *)&cnt->stats[kStatRevCells], (unsigned long long)n_rev);
@@ -307,14 +376,17 @@ TSI_DI void flush_tally(Counters *cnt, const LevelTally &t) {
         if (fwd_ops) atomicAdd((unsigned long long *)&cnt->stats[kStatFwdOps], (unsigned long long)fwd_ops);
         if (n_pruned) atomicAdd((unsigned long long *)&cnt->stats[kStatPruned], (unsigned long long)n_pruned);
         if (n_freed) atomicAdd((unsigned long long *)&cnt->stats[kStatDeallocs], (unsigned long long)n_freed);
+        if (n_spec) atomicAdd((unsigned long long *)&cnt->stats[kStatSpec], (unsigned long long)n_spec);
     }
 }
 
 // One level of one frontier list: wavefront `wave` of `nwaves` strides over the list; `bwave`
 // indexes the wavefront's own buffers (saved-operand stack, choice rows), unique in the launch.
+// level -1 (reverse list): the speculative pass over the candidate parents (eval_group).
 template <bool REV, class Engine>
 TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, uint32_t nwaves, uint32_t bwave_) {
-    const uint32_t count = list_entries(a, level, REV ? 1 : 0);
+    const bool sp = REV && level < 0;
+    const uint32_t count = sp ? min(*a.spec_in_count, a.spec_cap) : list_entries(a, level, REV ? 1 : 0);
     // wave indices made visibly uniform: the grid-stride loop and everything inside it then
     // compile as scalar control flow
     const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_);
@@ -325,7 +397,8 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
     // array, so that it is promoted to VGPRs; unused by straight-line engines)
     Slots Rlo, Rhi;
     LevelTally tl;
-    for (uint32_t g = wave; g * 8u < count; g += nwaves) eval_group<REV, Engine>(a, level, time, g, count, stk, E, Rlo, Rhi, tl);
+    for (uint32_t g = wave; g * 8u < count; g += nwaves)
+        eval_group<REV, Engine>(a, level, time, g, count, stk, E, Rlo, Rhi, tl, sp);
     // ---- the dealloc cascade's step for the blocks dropped by the previous level and their
     //      descendants (forward-only kernel; the reverse kernel of the level does not touch the
     //      list, which the previous level's kernels finished) ----
@@ -350,6 +423,7 @@ TSI_DI void frame_begin(const KArgs &a, uint32_t t, uint32_t nt) {
         c->alloc_tickets = 0;
         c->pending = 0;
         c->frame_error = 0;
+        if (a.spec_levels) *a.spec_out_count = 0;
         if (c->root_pending) {
             c->root_pending = 0;
             const float4 s = a.st[0];

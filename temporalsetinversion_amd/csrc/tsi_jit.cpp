@@ -871,7 +871,8 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
     // on its share of the grid (level_split), instead of one after the other
     o += "extern \"C\" __global__ void __launch_bounds__(256) tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n"
          "    const uint32_t nb = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6;\n"
-         "    const uint32_t nf = __builtin_amdgcn_readfirstlane(tsi::level_split(a, level, nb));\n"
+         "    // level -1: the frame's speculative pass, on the whole grid (reverse list code)\n"
+         "    const uint32_t nf = level < 0 ? 0u : __builtin_amdgcn_readfirstlane(tsi::level_split(a, level, nb));\n"
          "    const uint32_t bwave = b * 4u + w;\n"
          "    if (b < nf)\n"
          "        tsi::eval_level<false, tsi::jf::TapeEngine>(a, level, time, bwave, nf * 4u, bwave);\n"

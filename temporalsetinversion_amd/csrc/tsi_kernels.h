@@ -20,7 +20,7 @@ namespace tsi {
 constexpr int kMaxLevels = 24;
 enum : uint32_t { kErrQueue = 1u, kErrPool = 2u };
 enum { kStatCells = 0, kStatRevCells, kStatParents, kStatEnqueued, kStatRevOps, kStatDeallocs, kStatLive,
-       kStatFwdOps, kStatPruned, kNumStats = 10 };
+       kStatFwdOps, kStatPruned, kStatSpec, kNumStats = 10 };
 
 // Liveness record of a frontier entry (input of its children's pruned program), stored
 // word-major: word w of entry e at rec[w * prune_cap + e], so that a wavefront's consecutive
@@ -44,6 +44,19 @@ struct Counters {
     uint32_t pad2[2];
     uint64_t stats[kNumStats];
 };
+
+// A speculatively evaluated cell (speculative evaluation of the shallow levels, below): the
+// tape's results for one child of a candidate parent and the inputs they were computed from.
+// 64 B, one per lane: a wavefront writes 4 KB contiguous.
+struct SpecCell {
+    float field_lo, field_hi;
+    float sens[8];              // sensitivities of x, y, z, t: lo, hi pairs
+    float targ_lo, targ_hi;     // the t-window evaluated
+    uint32_t fc, ops;           // algorithmic forward / reverse ops
+    uint32_t flags;             // kSpec* below
+    uint32_t parent;            // the candidate parent's node
+};
+enum : uint32_t { kSpecEval = 1u, kSpecRev = 2u, kSpecPfast = 4u };
 
 struct KArgs {
     float4 *st, *blo, *bhi, *grad;
@@ -73,6 +86,19 @@ struct KArgs {
     uint32_t choice_len;
     FwdIns *pool;               // program pool: forward records then reverse records per group
     uint32_t pool_cap;          // records
+    // speculative evaluation of the shallow levels: the frontier parents of levels < spec_levels
+    // are kept as the next frame's candidates; at frame start one launch (level -1) evaluates
+    // the children of every candidate from the octree as the frame starts, and a level whose
+    // parent carries this frame's tag (bhi[P].w) takes a child's results from its record
+    // (blo[P].w) when the inputs match, instead of evaluating the cell again
+    uint32_t spec_levels;       // 0: off
+    uint32_t spec_cap;          // candidate parents per list
+    uint32_t spec_tag;          // this frame's tag (nonzero, new every frame)
+    const uint32_t *spec_in;    // the previous frame's candidates (evaluated at frame start)
+    const uint32_t *spec_in_count;
+    uint32_t *spec_out;         // this frame's candidates (the next frame's)
+    uint32_t *spec_out_count;
+    SpecCell *spec_res;         // 8 per candidate
 };
 
 struct TraceArgs {

@@ -670,6 +670,15 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 0);
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 1);
         }
+        if (compiled && d == 0 && a.spec_levels) {   // the speculative pass (level -1)
+            KArgs ka = a;
+            int level = -1;
+            float t = fl.time;
+            void *args[] = {&ka, &level, &t};
+            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, fl.grid_fwd + fl.grid_rev, 1, 1, 256, 1, 1, 0, s,
+                                      args, nullptr) != hipSuccess)
+                return -1;
+        }
         if (compiled) {
             KArgs ka = a;
             int level = d;

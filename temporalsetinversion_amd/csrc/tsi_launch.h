@@ -9,6 +9,9 @@
 
 namespace tsi {
 
+// levels evaluated speculatively at frame start unless $TSI_SPEC_LEVELS says otherwise
+constexpr long kSpecLevelsDefault = 7;
+
 // every TSI_SHADE_* bit; tsi_params.shading with any other bit is rejected
 constexpr uint32_t kShadeBits = TSI_SHADE_NORMALS | TSI_SHADE_GRADIENT | TSI_SHADE_CINEMATIC;
 

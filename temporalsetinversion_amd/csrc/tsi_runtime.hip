@@ -5,7 +5,9 @@
 // (649-656 rt_main, 683-690 buf_main).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -80,6 +82,12 @@ struct tsi_ctx {
     FwdIns *pool = nullptr;
     uint32_t prune_cap = 0, nchunks = 0, choice_len = 0, pool_cap = 0;
     bool prune = false;
+    // speculative evaluation of the shallow levels (KArgs::spec_*): candidate lists and their
+    // counts (two, swapped every frame), records, the last frame's tag, levels asked for
+    uint32_t *spec_par[2] = {nullptr, nullptr};
+    uint32_t *spec_cnt = nullptr;
+    SpecCell *spec_res = nullptr;
+    uint32_t spec_cap = 0, spec_tag = 0, spec_parity = 0, spec_max = 0;
     // compiled per-tape evaluation kernels
     hipModule_t mod[2] = {nullptr, nullptr};
     hipFunction_t eval_fn[2] = {nullptr, nullptr};   // forward-only list, reverse list
@@ -149,6 +157,12 @@ template <class T>
 void dfree(T *&p) {
     if (p) hipFree(p);
     p = nullptr;
+}
+
+// Whether frames evaluate their shallow levels speculatively (KArgs::spec_*): the compiled
+// engine, unpruned, and a depth that leaves a level to take the results.
+bool spec_active(const tsi_ctx *c) {
+    return c->spec_max > 0 && c->eval_fn[0] != nullptr && !c->prune && c->p.localsubdiv >= 2;
 }
 
 KArgs make_args(const tsi_ctx *c) {
@@ -344,6 +358,7 @@ void release(tsi_ctx *c) {
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
     dfree(c->pending); dfree(c->free_stack); dfree(c->cnt); dfree(c->stack);
+    dfree(c->spec_par[0]); dfree(c->spec_par[1]); dfree(c->spec_cnt); dfree(c->spec_res);
     dfree(c->fprog); dfree(c->rprog);
     for (auto &e : c->ev) if (e) { hipEventDestroy(e); e = nullptr; }
     if (c->stream) { hipStreamDestroy(c->stream); c->stream = nullptr; }
@@ -495,6 +510,18 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             for (auto &x : l)
                 if (!rc) rc = dmalloc(&x, c->list_cap);
         if (rc) break;
+        // speculative candidates: up to 2^16 parents (8 records of 64 B each: 32 MiB)
+        c->spec_cap = 1u << 16;
+        {
+            const char *e = std::getenv("TSI_SPEC_LEVELS");
+            const long v = e ? std::strtol(e, nullptr, 10) : kSpecLevelsDefault;
+            c->spec_max = (params->flags & TSI_FLAG_NO_SPECULATE) || v < 0 ? 0u : uint32_t(v < kMaxLevels ? v : kMaxLevels);
+        }
+        if (c->spec_max &&
+            ((rc = dmalloc(&c->spec_par[0], c->spec_cap)) || (rc = dmalloc(&c->spec_par[1], c->spec_cap)) ||
+             (rc = dmalloc(&c->spec_cnt, 2)) || (rc = dmalloc(&c->spec_res, size_t(c->spec_cap) * 8u))))
+            break;
+        if (c->spec_cnt && launch_fill_u32(c->spec_cnt, 0u, 2u, c->stream)) { rc = TSI_E_HIP; break; }
         if ((rc = dmalloc(&c->dlists[0], c->dlist_cap)) || (rc = dmalloc(&c->dlists[1], c->dlist_cap)) ||
             (rc = dmalloc(&c->pending, c->nblocks)) || (rc = dmalloc(&c->free_stack, c->nblocks)) ||
             (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->tslot[0].visits, 1)) ||
@@ -589,6 +616,8 @@ int tsi_reset(tsi_ctx *c, const float root_box[6]) {
     if (harvest_all(c)) return TSI_E_HIP;
     KArgs a = make_args(c);
     if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
+    // the previous octree's frontier is no candidate for the new one
+    if (c->spec_cnt && launch_fill_u32(c->spec_cnt, 0u, 2u, c->stream)) return TSI_E_HIP;
     HIP_OK(hipStreamSynchronize(c->stream));
     std::memcpy(c->root_box, root_box, sizeof c->root_box);
     c->t_seen_lo = 1.0f;
@@ -652,6 +681,19 @@ int tsi_set_inversion_async(tsi_ctx *c) {
     }
     tsi_ctx::FrameSlot &f = c->fslot[c->f_issued % tsi_ctx::kFrameSlots];
     KArgs a = make_args(c);
+    if (spec_active(c)) {
+        a.spec_levels = std::min<uint32_t>(c->spec_max, uint32_t(c->p.localsubdiv - 1));
+        a.spec_cap = c->spec_cap;
+        if (++c->spec_tag == 0u) c->spec_tag = 1u;
+        a.spec_tag = c->spec_tag;
+        const uint32_t in = c->spec_parity, out = in ^ 1u;
+        a.spec_in = c->spec_par[in];
+        a.spec_in_count = c->spec_cnt + in;
+        a.spec_out = c->spec_par[out];
+        a.spec_out_count = c->spec_cnt + out;
+        a.spec_res = c->spec_res;
+        c->spec_parity = out;
+    }
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
                    (void *)c->eval_fn[0]};
     HIP_OK(hipEventRecord(f.begin, c->stream));
@@ -688,13 +730,15 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         stats->rev_evals = h.stats[kStatRevCells];
         stats->fwd_ops = h.stats[kStatFwdOps];
         stats->pruned_instr = h.stats[kStatPruned];
+        stats->spec_cells = h.stats[kStatSpec];
         stats->rev_ops = h.stats[kStatRevOps];
         stats->parents = h.stats[kStatParents];
         stats->deallocs = h.stats[kStatDeallocs];
         stats->enqueued = h.stats[kStatEnqueued];
         stats->allocated_nodes = h.stats[kStatLive] * 8u;
-        // the compiled engine runs both lists of a level in one launch, the interpreter in two
-        stats->eval_launches = uint64_t((c->eval_fn[0] ? 1 : 2) * c->p.localsubdiv);
+        // the compiled engine runs both lists of a level in one launch (and the speculative pass
+        // in one more), the interpreter in two
+        stats->eval_launches = uint64_t((c->eval_fn[0] ? 1 : 2) * c->p.localsubdiv) + (spec_active(c) ? 1u : 0u);
         stats->prune_launches = c->prune && !c->eval_fn[0] ? uint64_t(2 * (c->p.localsubdiv - 1)) : 0u;
     }
     c->used_nodes = (uint64_t(h.bump) + 1u) * 8u;

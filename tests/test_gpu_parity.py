@@ -45,6 +45,38 @@ def test_param_variants(opts, engine):
         assert_same_nodes(g, o, f"{opts} frame {k}")
 
 
+@pytest.mark.parametrize("scene,L,kw", [("spheres", 6, {}), ("capsules", 5, dict(lipschitz_corrected=True)),
+                                       ("csg", 5, dict(swept_volume=True))])
+def test_speculative_levels_change_nothing(scene, L, kw):
+    """The frame-start speculative pass over the previous frame's shallow frontier parents (on by
+    default for the unpruned compiled engine): every frame's nodes and statistics equal those of
+    frames evaluated level by level, and from the second frame on the shallow levels take most
+    of their cells from it."""
+    tape = tape_of(scene)
+    runs = {}
+    for spec in (True, False):
+        out = []
+        with T.State(tape, localsubdiv=L, voxel_capacity=1 << 20, prune=False, speculate=spec, **kw) as st:
+            st.reset()
+            for k, t in enumerate(SEQ[:7]):
+                st.update(float(np.float32(t)))
+                out.append((st.set_inversion().stats, st.read_nodes()))
+                st.enqueue_root()
+        runs[spec] = out
+    taken = 0
+    for k, ((s1, n1), (s0, n0)) in enumerate(zip(runs[True], runs[False])):
+        print(f"{scene} frame {k}: cells {s1['cell_evals']}, from the speculative pass {s1['spec_cells']}", flush=True)
+        assert_same_nodes(n1, n0, f"{scene} frame {k}")
+        for key in ("cell_evals", "rev_evals", "fwd_ops", "rev_ops", "parents", "enqueued", "deallocs",
+                    "allocated_nodes"):
+            assert s1[key] == s0[key], f"frame {k}: {key}"
+        assert s0["spec_cells"] == 0
+        if k == 0:
+            assert s1["spec_cells"] == 0
+        taken += s1["spec_cells"]
+    assert taken > 0
+
+
 @pytest.mark.parametrize("engine", ["compiled-full", "interp-full"])
 def test_stats_match_oracle(engine):
     tape = tape_of("spheres")
