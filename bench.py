@@ -53,6 +53,10 @@ WORKLOADS = {
 DEFAULT_WORKLOAD = "csg256_L10_1920x1080"
 
 
+# the bench options of a profiled run: the headline workload's frames only
+PROFILE_ONLY = "--no-cpu --no-full --no-c1 --no-batched --no-served --no-cinematic"
+
+
 def _profile_path(workload: str) -> str:
     return os.path.join(ROOT, "profiles", f"tape_eval_{workload}.txt")
 
@@ -117,21 +121,21 @@ def summarize_profile(args) -> None:
              "#",
              "# commands (from /tmp on the GPU host, repo at $R, <d> = an output directory):",
              f"#   rocprofv3 --kernel-trace --stats -f csv -d <d>/kt -o kt -- python3 {R}/bench.py --steps 5 --warmup 2 "
-             f"--no-cpu --no-full --workload {W}"]
+             f"{PROFILE_ONLY} --workload {W}"]
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         lines.append(f"#   rocprofv3 --pmc {c} -f csv -d <d>/{c} -o pmc -- python3 {R}/bench.py --steps 2 --warmup 1 "
-                     f"--no-cpu --no-full --workload {W}")
+                     f"{PROFILE_ONLY} --workload {W}")
         lines.append(f"#   rocprofv3 --pmc {c} -f csv -d <d>/copy_{c} -o pmc -- python3 {R}/bench.py --calibrate-copy-mib "
                      f"{args.copy_mib}")
     if args.pmc_sq:
         lines.append(f"#   rocprofv3 --pmc <SQ counters of each pass below> -f csv -d <d>/SQ<n> -o pmc -- python3 {R}/bench.py "
-                     f"--steps 2 --warmup 1 --no-cpu --no-full --workload {W}")
+                     f"--steps 2 --warmup 1 {PROFILE_ONLY} --workload {W}")
     lines.append(f"#   python3 bench.py --workload {W} --skip-frames {skip} --summarize <d>/kt/..._kernel_trace.csv "
                  "--pmc-fetch <FETCH_SIZE counter_collection.csv> --pmc-write <WRITE_SIZE ...> --copy-fetch <...> "
                  "--copy-write <...> --pmc-sq <SQ pass csv files>")
     lines += ["#", f"# {frames} frames ({skip} warm-up, {frames - skip} timed); one tape-evaluation launch per octree "
               f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, after one more "
-              "launch at frame start (the speculative pass of the", "# shallow levels; not in frame 0), one snapshot + "
+              "launch at frame start (the speculative pass of the", "# shallow levels), one snapshot + "
               "trace per frame on a trace stream.", ""]
     lines.append(f"{'kernel':<40} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
     for name, (n, t) in sorted(per.items(), key=lambda x: -x[1][1]):
@@ -144,28 +148,37 @@ def summarize_profile(args) -> None:
             evals.setdefault(f, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     per_level = {}
     for f, ds in evals.items():
+        if len(ds) not in (L, L + 1):
+            continue   # another workload's frame
         labels = (["spec"] if len(ds) == L + 1 else []) + list(range(L))
         for lab, d in zip(labels, ds):
             per_level.setdefault(lab, []).append(d)
     if per_level:
         lines += ["", "tsi_tape_eval_level per octree level, timed frames (us): level  mean  min  max",
-                  "(spec: the frame-start speculative pass over the previous frame's shallow frontier)"]
+                  "(spec: the frame-start speculative pass over the octree's nodes at depths < 7, with its "
+                  "one-block node list k_spec_gather before it)"]
         for d in sorted(per_level, key=lambda x: -1 if x == "spec" else x):
             v = per_level[d]
             lines.append(f"  level {d:>4} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
     lines.append("")
     lines.append("timed frames only (the figures bench.py reads):")
-    for name in ("tsi_tape_eval_level", "tsi::k_trace"):
+    for name in ("tsi_tape_eval_level", "void tsi::k_trace<false>"):
         if name in timed:
             n, t = timed[name]
             lines.append(f"{name} timed {n} dispatches avg_us {t / n:.2f}")
     if args.pmc_fetch and args.pmc_write and args.copy_fetch and args.copy_write:
-        def per_kernel(path, counter):
-            """{kernel: [values per dispatch in dispatch order]} of one counter."""
-            out = {}
+        def per_kernel(path, counter, skip_frames=0):
+            """{kernel: [values per dispatch in dispatch order]} of one counter, leaving out the
+            dispatches of the first skip_frames frames (frames start at k_frame_begin)."""
+            out, frame = {}, -1
             for r in sorted(_csv_rows(path), key=lambda r: int(r["Dispatch_Id"])):
-                if r["Counter_Name"] == counter:
-                    out.setdefault(_kernel(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+                if r["Counter_Name"] != counter:
+                    continue
+                k = _kernel(r["Kernel_Name"])
+                if k == "tsi::k_frame_begin":
+                    frame += 1
+                if frame >= skip_frames or "copyBuffer" in k:
+                    out.setdefault(k, []).append(float(r["Counter_Value"]))
             return out
 
         kib = args.copy_mib * 1024.0
@@ -175,15 +188,14 @@ def summarize_profile(args) -> None:
             return kib / (sum(v) / len(v))
 
         sf, sw = scale(args.copy_fetch, "FETCH_SIZE"), scale(args.copy_write, "WRITE_SIZE")
-        f, w = per_kernel(args.pmc_fetch, "FETCH_SIZE"), per_kernel(args.pmc_write, "WRITE_SIZE")
-        # the counter runs have 1 warm-up frame: its L level launches are left out
+        # the counter runs have 1 warm-up frame: its dispatches are left out
+        f, w = per_kernel(args.pmc_fetch, "FETCH_SIZE", 1), per_kernel(args.pmc_write, "WRITE_SIZE", 1)
         lines += ["", f"copy calibration ({args.copy_mib} MiB device-to-device): FETCH_SIZE scale {sf:.3f}, "
                       f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024); level launches of the timed frames",
                   f"{'kernel':<24} {'':>4} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'bytes/disp':>15}"]
-        for k in ("tsi_tape_eval_level", "tsi::k_trace", "tsi::k_snapshot"):
+        for k in ("tsi_tape_eval_level", "void tsi::k_trace<false>", "tsi::k_snapshot"):
             if k in f and k in w:
-                cut = L if k == "tsi_tape_eval_level" else 1
-                vf, vw = f[k][cut:], w[k][cut:]
+                vf, vw = f[k], w[k]
                 if not vf or not vw:
                     continue
                 b = (sum(vf) / len(vf) * sf + sum(vw) / len(vw) * sw) * 1024.0
@@ -193,12 +205,16 @@ def summarize_profile(args) -> None:
         rows = _csv_rows(path)
         counters = sorted({r["Counter_Name"] for r in rows})
         vals = {}
-        for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
-            if _kernel(r["Kernel_Name"]) == "tsi_tape_eval_level":
+        frame = -1
+        for r in sorted(rows, key=lambda r: (int(r["Dispatch_Id"]), r["Counter_Name"])):
+            k = _kernel(r["Kernel_Name"])
+            if k == "tsi::k_frame_begin" and r["Counter_Name"] == counters[0]:
+                frame += 1
+            if k == "tsi_tape_eval_level" and frame >= 1:
                 vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
         lines += ["", f"SQ counters per tsi_tape_eval_level dispatch (timed frames), {os.path.basename(path)}:"]
         for c in counters:
-            v = vals.get(c, [])[L:]
+            v = vals.get(c, [])
             if v:
                 lines.append(f"  {c:<28} {sum(v) / len(v):>16.1f}")
     print("\n".join(lines))
@@ -211,6 +227,8 @@ def summarize_kernels(args) -> None:
     per = {}
     for r in rows:
         name = _kernel(r["Kernel_Name"])
+        if args.split_grid:   # one line per launch shape (threads per grid dimension)
+            name = name + " " + r["Grid_Size_X"] + "x" + r["Grid_Size_Y"]
         d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
         per.setdefault(name, []).append(d)
     total = sum(sum(v) for v in per.values())
@@ -218,7 +236,8 @@ def summarize_kernels(args) -> None:
     lines = [f"# Kernel time: {args.summary_title}", "#",
              "# command (from /tmp on the GPU host, repo at $R, <d> = an output directory):",
              f"#   rocprofv3 --kernel-trace --stats -f csv -d <d> -o kt -- python3 {R}/bench.py {args.summary_cmd}",
-             f"#   python3 bench.py --summarize-kernels <d>/kt_kernel_trace.csv --summary-title ... --summary-cmd ...",
+             f"#   python3 bench.py --summarize-kernels <d>/kt_kernel_trace.csv --summary-title ... --summary-cmd ..."
+             + (" --split-grid" if args.split_grid else ""),
              "", f"{'kernel':<44} {'dispatches':>10} {'total_ms':>10} {'avg_us':>11} {'min_us':>11} {'max_us':>11} {'share':>7}"]
     for name, v in sorted(per.items(), key=lambda x: -sum(x[1])):
         lines.append(f"{name[:44]:<44} {len(v):>10} {sum(v) / 1e3:>10.3f} {sum(v) / len(v):>11.1f} {min(v):>11.1f} "
@@ -700,9 +719,11 @@ def main():
                     help="instead of the benchmark: per-kernel summary of a rocprofv3 kernel trace")
     ap.add_argument("--summary-title", default="")
     ap.add_argument("--summary-cmd", default="")
+    ap.add_argument("--split-grid", action="store_true", help="with --summarize-kernels: a line per launch shape")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
     ap.add_argument("--no-served", action="store_true", help="skip the served form")
     ap.add_argument("--no-cinematic", action="store_true", help="skip the cinematic path tracer")
+    ap.add_argument("--no-c1", action="store_true", help="skip the reference's CPU configuration on the GPU")
     ap.add_argument("--no-tune", action="store_true", help="batched form: keep the default launch shape")
     ap.add_argument("--batch-requests", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -897,7 +918,8 @@ def main():
     }
     if tvf is not None:
         out["temporal_vs_full"] = tvf
-    out["gpu_c1"] = gpu_c1(T)
+    if not args.no_c1:
+        out["gpu_c1"] = gpu_c1(T)
     if not args.no_batched:
         print("batched form", file=sys.stderr, flush=True)
         out["batched"] = batched_form(T, args)

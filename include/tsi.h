@@ -107,12 +107,13 @@ enum {
     TSI_FLAG_INTERPRET = 4u,
     /* Do not evaluate the shallow levels speculatively.  By default the compiled engine, when
      * the tape is not pruned, evaluates at each frame's start, in one launch, the children of
-     * the previous frame's frontier parents of the first levels (up to 7, $TSI_SPEC_LEVELS
-     * sets another count, 0 none), where each level otherwise runs one wavefront's dependent
-     * chain through the tape on an idle device; a level then takes a cell's results from
-     * that launch when they were computed from the cell's inputs of this frame (same parent
-     * fast flag and t-window) and evaluates the others.  Results and statistics are the same
-     * either way; the cells whose parents are not enqueued this frame are extra work. */
+     * every node of the first levels (depth < 7, $TSI_SPEC_LEVELS sets another count, 0 none)
+     * as the octree stands (a leaf's children as new cells), where each of those levels
+     * otherwise runs one wavefront's dependent chain through the tape on an idle device; a
+     * level then takes a cell's results from that launch when they were computed from the
+     * cell's inputs of this frame (same box and t-window) and evaluates the others.  Results
+     * and statistics are the same either way; the cells of parents that are not enqueued this
+     * frame are extra work. */
     TSI_FLAG_NO_SPECULATE = 8u
 };
 

@@ -67,31 +67,19 @@ TSI_DI uint32_t list_entries(const KA &a, int level, int which) {
     return min(a.cnt->list_count[level][which], a.list_cap);
 }
 
-// Appends v for every lane with `want` to out[0 .. cap) (one atomic per wavefront); the count
-// keeps growing past cap, the entries beyond it are dropped.
-TSI_DI void append_capped(bool want, uint32_t v, uint32_t *out, uint32_t *count, uint32_t cap) {
-    const uint64_t m = ballot(want);
-    if (!m) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(count, popc64(m));
-    base = bcast(base, 0);
-    const uint32_t at = base + rank_in(m);
-    if (want && at < cap) out[at] = v;
-}
-
 // Group g (entries 8g .. 8g + 7) of one frontier list at one level: 8 parents x 8 child octants
 // on the wavefront's 64 lanes.  count = list_entries(a, level, REV); stk = the wavefront's
 // saved-operand stack (uniform base; lane l uses column l of each row).
 // spec (reverse list only): the speculative pass of the frame's start: the entries are the
-// candidate parents (a.spec_in), every child of a parent that has children is evaluated as the
-// octree stands, and only its SpecCell record and the parent's tag are written.
+// listed parents (a.spec_list), every child is evaluated as the octree stands (the children of
+// a leaf as new cells), and only its SpecCell record and the parent's tag are written.
 template <bool REV, class Engine, class KA>
 TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t count, float2 *stk, Engine &E,
                        Slots &Rlo, Slots &Rhi, LevelTally &tl, bool spec = false) {
     const int which = REV ? 1 : 0;
     const bool sp = REV && spec;
     Counters *cnt = a.cnt;
-    const uint32_t *list = sp ? a.spec_in : a.lists[level & 1][which];
+    const uint32_t *list = sp ? a.spec_list : a.lists[level & 1][which];
     uint32_t *next_f = a.lists[(level + 1) & 1][0];
     uint32_t *next_r = a.lists[(level + 1) & 1][1];
     const uint32_t lane = lane_id();
@@ -130,15 +118,15 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
                 }
             }
         }
-        bool ok = active && !(sp && isnew);
-        if (isnew && ok && blk >= a.nblocks) {
+        bool ok = active;
+        if (!sp && isnew && ok && blk >= a.nblocks) {
             ok = false;
             if (o == 0) atomicOr(&cnt->frame_error, kErrPool);
         }
-        if (isnew && ok && o == 0) a.st[P].w = __uint_as_float((pmeta & kFastBit) | (blk + 1u));
+        if (!sp && isnew && ok && o == 0) a.st[P].w = __uint_as_float((pmeta & kFastBit) | (blk + 1u));
         if (!sp) tl.par += (active && o == 0) ? 1 : 0;
 
-        const uint32_t C = ok ? blk * 8u + o : 0u;
+        const uint32_t C = ok && !(sp && isnew) ? blk * 8u + o : 0u;
         float4 cst = make_float4(1.0f, 0.0f, 0.0f, 0.0f);   // mallocV (compute.wgsl:3437-3449)
         if (ok && !isnew) cst = a.st[C];
         uint32_t cmeta = meta_of(cst);
@@ -150,8 +138,9 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
         Iv cx = I(plo.x + dx * float(lx), plo.x + dx * float(1u + lx));
         Iv cy = I(plo.y + dy * float(ly), plo.y + dy * float(1u + ly));
         Iv cz = I(plo.z + dz * float(lz), plo.z + dz * float(1u + lz));
-        // (a node keeps its box while it is allocated: only new nodes write it)
-        if (ok && isnew) {
+        // (a node keeps its box while it is allocated: only new nodes write it; the speculative
+        // pass writes no node)
+        if (!sp && ok && isnew) {
             a.blo[C] = make_float4(cx.lo, cy.lo, cz.lo, 0.0f);
             a.bhi[C] = make_float4(cx.hi, cy.hi, cz.hi, 0.0f);
         }
@@ -209,12 +198,14 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
         //      carries this frame's tag) from the same inputs ----
         bool have = false;
         const SpecCell *sr = nullptr;
-        if (!sp && level < int(a.spec_levels) && ok && !isnew && __float_as_uint(phi.w) == a.spec_tag) {
+        if (!sp && level < int(a.spec_levels) && ok && __float_as_uint(phi.w) == a.spec_tag) {
             const uint32_t j = __float_as_uint(plo.w);
             if (j < a.spec_cap) {
                 sr = a.spec_res + size_t(j) * 8u + o;
                 const uint32_t fl = sr->flags;
-                have = guard && sr->parent == P && (fl & kSpecEval) != 0 && ((fl & kSpecPfast) != 0) == pfast &&
+                // the tape's results depend only on the box (the parent's: the same node), the
+                // t-window and whether the reverse pass ran
+                have = guard && sr->parent == P && (fl & kSpecEval) != 0 &&
                        __float_as_uint(sr->targ_lo) == __float_as_uint(targ.lo) &&
                        __float_as_uint(sr->targ_hi) == __float_as_uint(targ.hi) && (!REV || !td || (fl & kSpecRev) != 0);
             }
@@ -261,7 +252,8 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
                     r.targ_hi = targ.hi;
                     r.fc = fcv;
                     r.ops = ops;
-                    r.flags = (guard ? kSpecEval : 0u) | (guard && td ? kSpecRev : 0u) | (pfast ? kSpecPfast : 0u);
+                    r.flags = (guard ? kSpecEval : 0u) | (guard && td ? kSpecRev : 0u) | (pfast ? kSpecPfast : 0u) |
+                              (isnew ? kSpecNew : 0u);
                     r.parent = P;
                     a.spec_res[size_t(g * 8u + ps) * 8u + o] = r;
                     if (o == 0) {
@@ -353,8 +345,6 @@ TSI_DI void eval_group(const KA &a, int level, float time, uint32_t g, uint32_t 
             if (!valid) rec = nullptr;
             if (ballot(rec != nullptr) != 0) E.liveness(a, rec);
         }
-        // this level's parents are the next frame's speculative candidates
-        if (level < int(a.spec_levels)) append_capped(active && o == 0, P, a.spec_out, a.spec_out_count, a.spec_cap);
         tl.cells += guard ? 1 : 0;
         tl.spec += have ? 1 : 0;
         tl.rev += (guard && td) ? 1 : 0;
@@ -386,7 +376,7 @@ TSI_DI void flush_tally(Counters *cnt, const LevelTally &t) {
 template <bool REV, class Engine>
 TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, uint32_t nwaves, uint32_t bwave_) {
     const bool sp = REV && level < 0;
-    const uint32_t count = sp ? min(*a.spec_in_count, a.spec_cap) : list_entries(a, level, REV ? 1 : 0);
+    const uint32_t count = sp ? min(*a.spec_count, a.spec_cap) : list_entries(a, level, REV ? 1 : 0);
     // wave indices made visibly uniform: the grid-stride loop and everything inside it then
     // compile as scalar control flow
     const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_);
@@ -423,7 +413,6 @@ TSI_DI void frame_begin(const KArgs &a, uint32_t t, uint32_t nt) {
         c->alloc_tickets = 0;
         c->pending = 0;
         c->frame_error = 0;
-        if (a.spec_levels) *a.spec_out_count = 0;
         if (c->root_pending) {
             c->root_pending = 0;
             const float4 s = a.st[0];

@@ -56,7 +56,7 @@ struct SpecCell {
     uint32_t flags;             // kSpec* below
     uint32_t parent;            // the candidate parent's node
 };
-enum : uint32_t { kSpecEval = 1u, kSpecRev = 2u, kSpecPfast = 4u };
+enum : uint32_t { kSpecEval = 1u, kSpecRev = 2u, kSpecPfast = 4u, kSpecNew = 8u };
 
 struct KArgs {
     float4 *st, *blo, *bhi, *grad;
@@ -86,19 +86,17 @@ struct KArgs {
     uint32_t choice_len;
     FwdIns *pool;               // program pool: forward records then reverse records per group
     uint32_t pool_cap;          // records
-    // speculative evaluation of the shallow levels: the frontier parents of levels < spec_levels
-    // are kept as the next frame's candidates; at frame start one launch (level -1) evaluates
-    // the children of every candidate from the octree as the frame starts, and a level whose
-    // parent carries this frame's tag (bhi[P].w) takes a child's results from its record
+    // speculative evaluation of the shallow levels: at frame start the nodes of the octree at
+    // depths < spec_levels are listed (k_spec_gather) and one launch (level -1) evaluates the
+    // children of each from the octree as the frame starts (a leaf's as new cells); a level
+    // whose parent carries this frame's tag (bhi[P].w) takes a child's results from its record
     // (blo[P].w) when the inputs match, instead of evaluating the cell again
     uint32_t spec_levels;       // 0: off
-    uint32_t spec_cap;          // candidate parents per list
+    uint32_t spec_cap;          // listed parents at most
     uint32_t spec_tag;          // this frame's tag (nonzero, new every frame)
-    const uint32_t *spec_in;    // the previous frame's candidates (evaluated at frame start)
-    const uint32_t *spec_in_count;
-    uint32_t *spec_out;         // this frame's candidates (the next frame's)
-    uint32_t *spec_out_count;
-    SpecCell *spec_res;         // 8 per candidate
+    uint32_t *spec_list;        // the listed parents (breadth first from the root)
+    uint32_t *spec_count;       // how many
+    SpecCell *spec_res;         // 8 per listed parent
 };
 
 struct TraceArgs {

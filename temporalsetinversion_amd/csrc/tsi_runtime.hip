@@ -82,12 +82,12 @@ struct tsi_ctx {
     FwdIns *pool = nullptr;
     uint32_t prune_cap = 0, nchunks = 0, choice_len = 0, pool_cap = 0;
     bool prune = false;
-    // speculative evaluation of the shallow levels (KArgs::spec_*): candidate lists and their
-    // counts (two, swapped every frame), records, the last frame's tag, levels asked for
-    uint32_t *spec_par[2] = {nullptr, nullptr};
+    // speculative evaluation of the shallow levels (KArgs::spec_*): the listed parents, their
+    // count, the records, the last frame's tag, the levels asked for
+    uint32_t *spec_list = nullptr;
     uint32_t *spec_cnt = nullptr;
     SpecCell *spec_res = nullptr;
-    uint32_t spec_cap = 0, spec_tag = 0, spec_parity = 0, spec_max = 0;
+    uint32_t spec_cap = 0, spec_tag = 0, spec_max = 0;
     // compiled per-tape evaluation kernels
     hipModule_t mod[2] = {nullptr, nullptr};
     hipFunction_t eval_fn[2] = {nullptr, nullptr};   // forward-only list, reverse list
@@ -358,7 +358,7 @@ void release(tsi_ctx *c) {
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
     dfree(c->pending); dfree(c->free_stack); dfree(c->cnt); dfree(c->stack);
-    dfree(c->spec_par[0]); dfree(c->spec_par[1]); dfree(c->spec_cnt); dfree(c->spec_res);
+    dfree(c->spec_list); dfree(c->spec_cnt); dfree(c->spec_res);
     dfree(c->fprog); dfree(c->rprog);
     for (auto &e : c->ev) if (e) { hipEventDestroy(e); e = nullptr; }
     if (c->stream) { hipStreamDestroy(c->stream); c->stream = nullptr; }
@@ -510,7 +510,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             for (auto &x : l)
                 if (!rc) rc = dmalloc(&x, c->list_cap);
         if (rc) break;
-        // speculative candidates: up to 2^16 parents (8 records of 64 B each: 32 MiB)
+        // speculative pass: up to 2^16 listed parents (8 records of 64 B each: 32 MiB)
         c->spec_cap = 1u << 16;
         {
             const char *e = std::getenv("TSI_SPEC_LEVELS");
@@ -518,10 +518,9 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             c->spec_max = (params->flags & TSI_FLAG_NO_SPECULATE) || v < 0 ? 0u : uint32_t(v < kMaxLevels ? v : kMaxLevels);
         }
         if (c->spec_max &&
-            ((rc = dmalloc(&c->spec_par[0], c->spec_cap)) || (rc = dmalloc(&c->spec_par[1], c->spec_cap)) ||
-             (rc = dmalloc(&c->spec_cnt, 2)) || (rc = dmalloc(&c->spec_res, size_t(c->spec_cap) * 8u))))
+            ((rc = dmalloc(&c->spec_list, c->spec_cap)) || (rc = dmalloc(&c->spec_cnt, 1)) ||
+             (rc = dmalloc(&c->spec_res, size_t(c->spec_cap) * 8u))))
             break;
-        if (c->spec_cnt && launch_fill_u32(c->spec_cnt, 0u, 2u, c->stream)) { rc = TSI_E_HIP; break; }
         if ((rc = dmalloc(&c->dlists[0], c->dlist_cap)) || (rc = dmalloc(&c->dlists[1], c->dlist_cap)) ||
             (rc = dmalloc(&c->pending, c->nblocks)) || (rc = dmalloc(&c->free_stack, c->nblocks)) ||
             (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->tslot[0].visits, 1)) ||
@@ -616,8 +615,6 @@ int tsi_reset(tsi_ctx *c, const float root_box[6]) {
     if (harvest_all(c)) return TSI_E_HIP;
     KArgs a = make_args(c);
     if (launch_reset(a, root_box, 8, c->stream) != 0) return TSI_E_HIP;
-    // the previous octree's frontier is no candidate for the new one
-    if (c->spec_cnt && launch_fill_u32(c->spec_cnt, 0u, 2u, c->stream)) return TSI_E_HIP;
     HIP_OK(hipStreamSynchronize(c->stream));
     std::memcpy(c->root_box, root_box, sizeof c->root_box);
     c->t_seen_lo = 1.0f;
@@ -686,13 +683,9 @@ int tsi_set_inversion_async(tsi_ctx *c) {
         a.spec_cap = c->spec_cap;
         if (++c->spec_tag == 0u) c->spec_tag = 1u;
         a.spec_tag = c->spec_tag;
-        const uint32_t in = c->spec_parity, out = in ^ 1u;
-        a.spec_in = c->spec_par[in];
-        a.spec_in_count = c->spec_cnt + in;
-        a.spec_out = c->spec_par[out];
-        a.spec_out_count = c->spec_cnt + out;
+        a.spec_list = c->spec_list;
+        a.spec_count = c->spec_cnt;
         a.spec_res = c->spec_res;
-        c->spec_parity = out;
     }
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
                    (void *)c->eval_fn[0]};
