@@ -85,6 +85,19 @@ uint32_t prefetch_regions() {
     return d;
 }
 
+// Minimum wavefronts per SIMD the level kernel is compiled for ($TSI_JIT_WAVES, 0: the
+// compiler's choice).  Default 4, i.e. at most 128 VGPRs: the 3 685-record CSG module then
+// spills 4 VGPRs (it held 167, 3 wavefronts per SIMD) and its frames run 4-5 % faster; 5
+// wavefronts (96 VGPRs) spill 41 for no further gain.
+uint32_t level_waves() {
+    static const uint32_t w = [] {
+        const char *e = std::getenv("TSI_JIT_WAVES");
+        const long v = e ? std::strtol(e, nullptr, 10) : 4;
+        return v >= 1 && v <= 8 ? uint32_t(v) : 0u;
+    }();
+    return w;
+}
+
 std::string V(uint32_t j) { return "v" + std::to_string(j); }
 std::string S(uint32_t j) { return "s" + std::to_string(j); }
 // row pos of the wavefront's stack, this lane's column (uniform row base + lane offset)
@@ -869,7 +882,8 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
     o += "}  // namespace jr\n}  // namespace tsi\n\n";
     // one launch per level: the forward-only list and the reverse list run side by side, each
     // on its share of the grid (level_split), instead of one after the other
-    o += "extern \"C\" __global__ void __launch_bounds__(256) tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n"
+    const std::string lb = level_waves() ? "256, " + std::to_string(level_waves()) : "256";
+    o += "extern \"C\" __global__ void __launch_bounds__(" + lb + ") tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n"
          "    const uint32_t nb = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6;\n"
          "    // level -1: the frame's speculative pass, on the whole grid (reverse list code)\n"
          "    const uint32_t nf = level < 0 ? 0u : __builtin_amdgcn_readfirstlane(tsi::level_split(a, level, nb));\n"
