@@ -702,7 +702,22 @@ int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s) {
 
 int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1,
                  const hipEvent_t *e_level) {
+    const SpecLaunch *sp = fl.tape_eval ? fl.spec : nullptr;
+    if (sp) {   // the speculative pass, beside the previous frame's last levels
+        if (sp->wait) hipStreamWaitEvent(sp->stream, sp->wait, 0);
+        hipEventRecord(sp->begin, sp->stream);
+        hipLaunchKernelGGL(k_spec_gather, dim3(1), dim3(1024), 0, sp->stream, sp->args);
+        KArgs ka = sp->args;
+        int level = -1;
+        float t = fl.time;
+        void *args[] = {&ka, &level, &t};
+        if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, sp->blocks, 1, 1, 256, 1, 1, 0, sp->stream, args,
+                                  nullptr) != hipSuccess)
+            return -1;
+        hipEventRecord(sp->done, sp->stream);
+    }
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a);
+    if (sp) hipStreamWaitEvent(s, sp->done, 0);
     if (e_eval0) hipEventRecord(e_eval0, s);
     for (int d = 0; d < fl.levels; ++d) {
         const bool compiled = fl.tape_eval != nullptr;
@@ -710,16 +725,6 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
         if (a.prune && d > 0 && !compiled) {   // the interpreter's group programs
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 0);
             hipLaunchKernelGGL(k_prune, dim3(fl.grid_misc), dim3(256), 0, s, a, d, 1);
-        }
-        if (compiled && d == 0 && a.spec_levels) {   // the speculative pass (level -1)
-            hipLaunchKernelGGL(k_spec_gather, dim3(1), dim3(1024), 0, s, a);
-            KArgs ka = a;
-            int level = -1;
-            float t = fl.time;
-            void *args[] = {&ka, &level, &t};
-            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, fl.grid_fwd + fl.grid_rev, 1, 1, 256, 1, 1, 0, s,
-                                      args, nullptr) != hipSuccess)
-                return -1;
         }
         if (compiled) {
             KArgs ka = a;
@@ -735,6 +740,8 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
             hipLaunchKernelGGL(k_eval<true>, dim3(fl.grid_rev), dim3(256), 0, s, a, d, fl.time);
         }
         if (e_level) hipEventRecord(e_level[2 * d + 1], s);
+        if (sp && d == (sp->ready_level < fl.levels - 1 ? sp->ready_level : fl.levels - 1))
+            hipEventRecord(sp->ready, s);
     }
     if (e_eval1) hipEventRecord(e_eval1, s);
     // levels 1..L-1 ran the cascade's steps 1..L-1 (eval_level); blocks dropped by the last

@@ -15,11 +15,25 @@ constexpr long kSpecLevelsDefault = 7;
 // every TSI_SHADE_* bit; tsi_params.shading with any other bit is rejected
 constexpr uint32_t kShadeBits = TSI_SHADE_NORMALS | TSI_SHADE_GRADIENT | TSI_SHADE_CINEMATIC;
 
+// The frame's speculative pass (KArgs::spec_*) on a stream of its own: it waits for `wait` (the
+// previous frame's levels up to the deepest node it reads, or null), runs k_spec_gather and the
+// level kernel at level -1 with `args` (its own saved-operand stack) on `blocks` blocks between
+// `begin` and `done`; level 0 of the frame waits for `done`.  The frame records `ready` after
+// its level `ready_level`, for the next frame's pass.
+struct SpecLaunch {
+    hipStream_t stream;
+    hipEvent_t wait, begin, done, ready;
+    KArgs args;
+    uint32_t blocks;
+    int ready_level;
+};
+
 struct FrameLaunch {
     int levels;
     float time;
     uint32_t grid_fwd, grid_rev, grid_misc;
     void *tape_eval;   // the tape's compiled level kernel (hipFunction_t), or null: k_eval
+    const SpecLaunch *spec;   // or null
 };
 
 // e_eval0 / e_eval1 (may be null) bracket the level loop; e_level (may be null) holds an event

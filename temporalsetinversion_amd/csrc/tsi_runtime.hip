@@ -88,6 +88,13 @@ struct tsi_ctx {
     uint32_t *spec_cnt = nullptr;
     SpecCell *spec_res = nullptr;
     uint32_t spec_cap = 0, spec_tag = 0, spec_max = 0;
+    // the pass runs on its own stream with its own saved-operand stack (one block per CU), after
+    // the previous frame's levels up to spec_max have run (spec_ready, recorded by that frame)
+    hipStream_t spec_stream = nullptr;
+    hipEvent_t spec_ready = nullptr;
+    bool spec_ready_recorded = false;
+    float2 *spec_stack = nullptr;
+    size_t spec_stack_bytes = 0;
     // compiled per-tape evaluation kernels
     hipModule_t mod[2] = {nullptr, nullptr};
     hipFunction_t eval_fn[2] = {nullptr, nullptr};   // forward-only list, reverse list
@@ -107,6 +114,8 @@ struct tsi_ctx {
     struct FrameSlot {
         hipEvent_t begin = nullptr, end = nullptr, eval0 = nullptr, eval1 = nullptr;
         hipEvent_t level[2 * kMaxLevels] = {};   // around each level's evaluation launches
+        hipEvent_t spec0 = nullptr, spec1 = nullptr;   // around the speculative pass
+        bool spec = false;                             // the frame had one
         Counters *h = nullptr;
     };
     // a trace runs on its slot's stream over its slot's device-side snapshot of the octree, so it
@@ -335,13 +344,14 @@ int load_jit(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n, const std:
 }
 
 void release(tsi_ctx *c) {
+    if (c->spec_stream) hipStreamSynchronize(c->spec_stream);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &t : c->tslot)
         if (t.stream) hipStreamSynchronize(t.stream);
     release_jit(c);
     release_prune(c);
     for (auto &f : c->fslot) {
-        for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1})
+        for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1, &f.spec0, &f.spec1})
             if (*e) { hipEventDestroy(*e); *e = nullptr; }
         for (hipEvent_t &e : f.level)
             if (e) { hipEventDestroy(e); e = nullptr; }
@@ -358,7 +368,9 @@ void release(tsi_ctx *c) {
     for (auto &l : c->lists) for (auto &x : l) dfree(x);
     dfree(c->dlists[0]); dfree(c->dlists[1]);
     dfree(c->pending); dfree(c->free_stack); dfree(c->cnt); dfree(c->stack);
-    dfree(c->spec_list); dfree(c->spec_cnt); dfree(c->spec_res);
+    dfree(c->spec_list); dfree(c->spec_cnt); dfree(c->spec_res); dfree(c->spec_stack);
+    if (c->spec_ready) { hipEventDestroy(c->spec_ready); c->spec_ready = nullptr; }
+    if (c->spec_stream) { hipStreamDestroy(c->spec_stream); c->spec_stream = nullptr; }
     dfree(c->fprog); dfree(c->rprog);
     for (auto &e : c->ev) if (e) { hipEventDestroy(e); e = nullptr; }
     if (c->stream) { hipStreamDestroy(c->stream); c->stream = nullptr; }
@@ -473,7 +485,9 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { rc = TSI_E_HIP; break; }
         c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         if ((rc = scratch_setup(c->device, &c->scratch_rgba8))) break;
-        bool sok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+        bool sok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+                   hipStreamCreateWithFlags(&c->spec_stream, hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&c->spec_ready, hipEventDisableTiming) == hipSuccess;
         for (auto &t : c->tslot)
             sok = sok && hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking) == hipSuccess &&
                   hipEventCreateWithFlags(&t.snap, hipEventDisableTiming) == hipSuccess;
@@ -484,7 +498,8 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         bool evok = true;
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         for (auto &f : c->fslot) {
-            for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1}) evok = evok && hipEventCreate(e) == hipSuccess;
+            for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1, &f.spec0, &f.spec1})
+                evok = evok && hipEventCreate(e) == hipSuccess;
             for (hipEvent_t &e : f.level) evok = evok && hipEventCreate(&e) == hipSuccess;
             evok = evok && hipHostMalloc((void **)&f.h, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
         }
@@ -586,6 +601,15 @@ int tsi_upload_tape(tsi_ctx *c, const tsi_gpu_instruction *tape, uint32_t n) {
         if ((rc = dmalloc(&c->stack, need / sizeof(float2)))) return rc;
         c->stack_bytes = need;
     }
+    // the speculative pass's own stack: one block of 4 wavefronts per CU
+    const size_t spec_need = c->spec_max ? size_t(c->num_cu) * 4u * 64u * size_t(L.stack + 1) * sizeof(float2) : 0;
+    if (spec_need > c->spec_stack_bytes) {
+        HIP_OK(hipStreamSynchronize(c->spec_stream));
+        dfree(c->spec_stack);
+        c->spec_stack_bytes = 0;
+        if ((rc = dmalloc(&c->spec_stack, spec_need / sizeof(float2)))) return rc;
+        c->spec_stack_bytes = spec_need;
+    }
     if ((rc = alloc_prune(c, L, fwd))) return rc;
     std::memset(&c->info, 0, sizeof c->info);
     c->info.n = n;
@@ -678,7 +702,9 @@ int tsi_set_inversion_async(tsi_ctx *c) {
     }
     tsi_ctx::FrameSlot &f = c->fslot[c->f_issued % tsi_ctx::kFrameSlots];
     KArgs a = make_args(c);
-    if (spec_active(c)) {
+    SpecLaunch sl{};
+    f.spec = spec_active(c);
+    if (f.spec) {
         a.spec_levels = std::min<uint32_t>(c->spec_max, uint32_t(c->p.localsubdiv - 1));
         a.spec_cap = c->spec_cap;
         if (++c->spec_tag == 0u) c->spec_tag = 1u;
@@ -686,9 +712,21 @@ int tsi_set_inversion_async(tsi_ctx *c) {
         a.spec_list = c->spec_list;
         a.spec_count = c->spec_cnt;
         a.spec_res = c->spec_res;
+        // the pass reads the octree down to depth spec_levels: it starts once the previous frame
+        // has run its levels 0 .. spec_max (its last levels run beside it)
+        sl.stream = c->spec_stream;
+        sl.wait = c->spec_ready_recorded ? c->spec_ready : nullptr;
+        sl.begin = f.spec0;
+        sl.done = f.spec1;
+        sl.ready = c->spec_ready;
+        sl.args = a;
+        sl.args.stack = c->spec_stack;
+        sl.blocks = uint32_t(c->num_cu);
+        sl.ready_level = int(c->spec_max);
+        c->spec_ready_recorded = true;
     }
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
-                   (void *)c->eval_fn[0]};
+                   (void *)c->eval_fn[0], f.spec ? &sl : nullptr};
     HIP_OK(hipEventRecord(f.begin, c->stream));
     if (launch_frame(a, fl, c->stream, f.eval0, f.eval1, f.level) != 0) return TSI_E_HIP;
     HIP_OK(hipMemcpyAsync(f.h, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
@@ -716,6 +754,11 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         for (int d = 0; d < c->p.localsubdiv; ++d) {
             float x = 0.0f;
             hipEventElapsedTime(&x, f.level[2 * d], f.level[2 * d + 1]);
+            kms += x;
+        }
+        if (f.spec) {   // the speculative pass (on its own stream, beside the previous frame)
+            float x = 0.0f;
+            hipEventElapsedTime(&x, f.spec0, f.spec1);
             kms += x;
         }
         stats->eval_kernel_ms = kms;
