@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "tsi.h"
 #include "tsi_eval.h"
 #include "tsi_interp.h"
@@ -700,6 +702,17 @@ int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Blocks (4 wavefronts, 8 entries each) a list of level d can fill: it holds at most the 8^d nodes
+// at depth d (and its dealloc list as many blocks), so the top levels launch a few blocks instead
+// of the full grid (whose idle blocks would wait for dispatch behind a trace running beside).
+uint32_t level_blocks(int d, uint32_t full) {
+    if (d >= 7) return full;
+    const uint64_t nodes = uint64_t(1) << (3 * d);
+    const uint64_t groups = (nodes + 7u) / 8u;
+    const uint64_t blocks = (groups + 3u) / 4u;
+    return blocks < full ? uint32_t(blocks) : full;
+}
+
 int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_t e_eval0, hipEvent_t e_eval1,
                  const hipEvent_t *e_level) {
     const SpecLaunch *sp = fl.tape_eval ? fl.spec : nullptr;
@@ -731,13 +744,15 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
             int level = d;
             float t = fl.time;
             void *args[] = {&ka, &level, &t};
-            // both lists of the level in one launch (tsi_tape_eval_level splits the grid)
-            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, fl.grid_fwd + fl.grid_rev, 1, 1, 256, 1, 1, 0, s,
-                                      args, nullptr) != hipSuccess)
+            // both lists of the level in one launch (tsi_tape_eval_level splits the grid: each
+            // list gets at least one block)
+            const uint32_t nb = std::min(fl.grid_fwd + fl.grid_rev, 2u * level_blocks(d, fl.grid_rev));
+            if (hipModuleLaunchKernel((hipFunction_t)fl.tape_eval, nb, 1, 1, 256, 1, 1, 0, s, args, nullptr) !=
+                hipSuccess)
                 return -1;
         } else {
-            hipLaunchKernelGGL(k_eval<false>, dim3(fl.grid_fwd), dim3(256), 0, s, a, d, fl.time);
-            hipLaunchKernelGGL(k_eval<true>, dim3(fl.grid_rev), dim3(256), 0, s, a, d, fl.time);
+            hipLaunchKernelGGL(k_eval<false>, dim3(level_blocks(d, fl.grid_fwd)), dim3(256), 0, s, a, d, fl.time);
+            hipLaunchKernelGGL(k_eval<true>, dim3(level_blocks(d, fl.grid_rev)), dim3(256), 0, s, a, d, fl.time);
         }
         if (e_level) hipEventRecord(e_level[2 * d + 1], s);
         if (sp && d == (sp->ready_level < fl.levels - 1 ? sp->ready_level : fl.levels - 1))
