@@ -134,29 +134,40 @@ def summarize_profile(args) -> None:
                  "--pmc-fetch <FETCH_SIZE counter_collection.csv> --pmc-write <WRITE_SIZE ...> --copy-fetch <...> "
                  "--copy-write <...> --pmc-sq <SQ pass csv files>")
     lines += ["#", f"# {frames} frames ({skip} warm-up, {frames - skip} timed); one tape-evaluation launch per octree "
-              f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, after one more "
-              "launch at frame start (the speculative pass of the", "# shallow levels), one snapshot + "
-              "trace per frame on a trace stream.", ""]
+              f"level ({L} per frame) running the", "# forward-only and the reverse list side by side, one more "
+              "launch per frame (the speculative pass of the next", "# frame's shallow levels, on its own stream), one "
+              "snapshot + trace per frame on a trace stream.", ""]
     lines.append(f"{'kernel':<40} {'dispatches':>10} {'total_ms':>10} {'avg_us':>9} {'share':>7}")
     for name, (n, t) in sorted(per.items(), key=lambda x: -x[1][1]):
         lines.append(f"{name[:40]:<40} {n:>10} {t / 1e3:>10.3f} {t / n:>9.2f} {100 * t / total:>6.1f}%")
-    # per octree level: the k-th evaluation launch of a frame runs level k, after the frame's
-    # speculative pass when the frame has one launch more than levels
-    evals = {}
-    for r, f in zip(rows, frame_of):
+    # per octree level: the k-th evaluation launch of a frame on the frame's queue runs level k;
+    # the speculative pass is the evaluation launch that follows k_spec_gather on its own queue
+    # (it runs beside the previous frame's last levels)
+    spec_rows, pending = set(), set()
+    for i, r in enumerate(rows):
+        k, q = _kernel(r["Kernel_Name"]), r["Queue_Id"]
+        if k == "tsi::k_spec_gather":
+            pending.add(q)
+        elif k == "tsi_tape_eval_level" and q in pending:
+            pending.discard(q)
+            spec_rows.add(i)
+    evals, per_level = {}, {}
+    for i, (r, f) in enumerate(zip(rows, frame_of)):
         if _kernel(r["Kernel_Name"]) == "tsi_tape_eval_level" and f >= skip:
-            evals.setdefault(f, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
-    per_level = {}
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            if i in spec_rows:
+                per_level.setdefault("spec", []).append(d)
+            else:
+                evals.setdefault(f, []).append(d)
     for f, ds in evals.items():
-        if len(ds) not in (L, L + 1):
+        if len(ds) != L:
             continue   # another workload's frame
-        labels = (["spec"] if len(ds) == L + 1 else []) + list(range(L))
-        for lab, d in zip(labels, ds):
+        for lab, d in zip(range(L), ds):
             per_level.setdefault(lab, []).append(d)
     if per_level:
         lines += ["", "tsi_tape_eval_level per octree level, timed frames (us): level  mean  min  max",
-                  "(spec: the frame-start speculative pass over the octree's nodes at depths < 7, with its "
-                  "one-block node list k_spec_gather before it)"]
+                  "(spec: the speculative pass over the octree's nodes at depths < 7 for the next frame, on a "
+                  "stream of its own beside levels 8-9, after its one-block node list k_spec_gather)"]
         for d in sorted(per_level, key=lambda x: -1 if x == "spec" else x):
             v = per_level[d]
             lines.append(f"  level {d:>4} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
