@@ -7,7 +7,10 @@
 // resident for the whole run (a cooperative launch: every block on a CU at once) and walks the
 // octree levels of all sessions together:
 //
-//   phase "begin"      every session's frame_begin (block-strided over sessions)
+//   phase "begin"      every session's frame_begin and the node list of its speculative pass
+//                      (spec_gather; block-strided over sessions)
+//   phase -1           the speculative pass: groups of 8 listed nodes of every session
+//                      (eval_group's speculative mode), when the sessions have one
 //   phase d = 0 .. D+1 work units of level d, claimed by wavefronts from a counter until none
 //                      is left: for each session, groups of 8 entries of its reverse list, of its
 //                      forward-only list (levels below its depth), and of its dealloc list
@@ -58,6 +61,7 @@ struct BatchArgs {
     uint32_t stack_len;
     uint32_t nsess;
     int32_t phases;         // levels + 2 (the deepest session's depth, then the cascade's tail)
+    int32_t first_phase;    // -1: the sessions evaluate their shallow levels speculatively, else 0
     uint32_t claim_units;   // units a wavefront takes per claim
     uint64_t wait_ticks;    // longest wait at a grid barrier, in s_memrealtime ticks (100 MHz)
 };
@@ -97,9 +101,14 @@ TSI_DI bool grid_sync(const BatchArgs &b, uint32_t &gen) {
     return ok_sh != 0u;
 }
 
-// Work units of session s in phase d: reverse groups, forward-only groups, dealloc groups.
+// Work units of session s in phase d: reverse groups, forward-only groups, dealloc groups
+// (phase -1: the speculative pass's groups, as reverse units).
 TSI_DI void batch_units(const BatchSession &S, int d, uint32_t &nr, uint32_t &nf, uint32_t &nd) {
     nr = nf = nd = 0u;
+    if (d < 0) {
+        if (S.a.spec_levels) nr = (min(*S.a.spec_count, S.a.spec_cap) + 7u) / 8u;
+        return;
+    }
     if (d < S.levels) {
         nr = (list_entries(S.a, d, 1) + 7u) / 8u;
         nf = (list_entries(S.a, d, 0) + 7u) / 8u;
@@ -158,6 +167,8 @@ TSI_DI void batch_run(const BatchArgs &b) {
         }
         __syncthreads();
         frame_begin(S.a, tid, blockDim.x);
+        if (S.a.spec_levels) spec_gather(S.a, tid, blockDim.x);
+        __syncthreads();
     }
     if (!grid_sync(b, gen)) return;
 
@@ -171,7 +182,7 @@ TSI_DI void batch_run(const BatchArgs &b) {
     // w * nb + i), so that a phase of a few units runs on as many CUs
     const uint32_t slot = __builtin_amdgcn_readfirstlane((tid >> 6) * nb + blockIdx.x);
 
-    for (int d = 0; d < b.phases; ++d) {
+    for (int d = b.first_phase; d < b.phases; ++d) {
         // ---- the phase's unit table (every block builds the same one) ----
         for (uint32_t s = tid; s < ns; s += blockDim.x) {
             uint32_t nr, nf, nd;
@@ -218,7 +229,8 @@ TSI_DI void batch_run(const BatchArgs &b) {
                 const float time = S.time;
                 LevelTally tl;
                 if (kind == 0u)
-                    Tapes::template group<true>(tape, a, d, time, g, list_entries(a, d, 1), stk, tl);
+                    Tapes::template group<true>(tape, a, d, time, g,
+                                                d < 0 ? min(*a.spec_count, a.spec_cap) : list_entries(a, d, 1), stk, tl);
                 else if (kind == 1u)
                     Tapes::template group<false>(tape, a, d, time, g, list_entries(a, d, 0), stk, tl);
                 else
@@ -227,7 +239,7 @@ TSI_DI void batch_run(const BatchArgs &b) {
             }
             if (nwaves >= total) break;
             uint32_t next = 0;
-            if (lane == 0) next = atomicAdd(&b.ctl->claim[d], cu);
+            if (lane == 0) next = atomicAdd(&b.ctl->claim[d + 1], cu);
             base = __builtin_amdgcn_readfirstlane(bcast(next, 0)) + nwaves;
         }
         if (!grid_sync(b, gen)) return;

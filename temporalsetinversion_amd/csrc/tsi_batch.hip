@@ -57,7 +57,9 @@ struct Session {
 
 // A session's arena layout: offsets of its arrays from the region's start.
 struct Layout {
-    uint64_t st, blo, bhi, grad, lists[4], dlists[2], pending, free_stack, cnt, rgba, depth, total;
+    uint64_t st, blo, bhi, grad, lists[4], dlists[2], pending, free_stack, cnt, rgba, depth;
+    uint64_t spec_list, spec_cnt, spec_res;   // the speculative pass's (spec_cap > 0)
+    uint64_t total;
 };
 
 uint64_t up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -75,7 +77,7 @@ bool session_sizes(const tsi_session_desc &d, uint64_t &nodes, uint32_t &list_ca
     return true;
 }
 
-Layout layout(uint64_t nodes, uint32_t list_cap, size_t px) {
+Layout layout(uint64_t nodes, uint32_t list_cap, size_t px, uint32_t spec_cap) {
     Layout L{};
     uint64_t o = 0;
     auto take = [&](uint64_t bytes) { const uint64_t at = o; o = up(o + bytes); return at; };
@@ -91,6 +93,11 @@ Layout layout(uint64_t nodes, uint32_t list_cap, size_t px) {
     L.cnt = take(sizeof(Counters));
     L.rgba = take(uint64_t(px) * 16);
     L.depth = take(uint64_t(px) * 4);
+    if (spec_cap) {
+        L.spec_list = take(uint64_t(spec_cap) * 4);
+        L.spec_cnt = take(4);
+        L.spec_res = take(uint64_t(spec_cap) * 8u * sizeof(SpecCell));
+    }
     L.total = o;
     return L;
 }
@@ -113,6 +120,9 @@ struct tsi_batch {
     std::vector<std::vector<tsi_gpu_instruction>> tapes;
     std::vector<Lowered> low;
     uint32_t grid = 0, stack_rows = 0, max_bpc = 1, wps = 1;
+    // speculative pass of the sessions' shallow levels (KArgs::spec_*): levels asked for
+    // ($TSI_SPEC_LEVELS), listed nodes per session, the last submit's tag
+    uint32_t spec_max = 0, spec_cap = 0, spec_tag = 0;
     float2 *stack = nullptr;
     BatchSession *sess_d = nullptr, *sess_h = nullptr;
     BatchCtl *ctl = nullptr;
@@ -245,6 +255,13 @@ void session_args(const tsi_batch *b, Session &s, const Layout &L) {
     a.fastenabled = s.p.fastenabled;
     a.force_deriv = (s.p.shading & (TSI_SHADE_NORMALS | TSI_SHADE_CINEMATIC)) && (s.p.shading & TSI_SHADE_GRADIENT) ? 1 : 0;
     a.prune = 0;
+    if (b->spec_max && s.p.localsubdiv >= 2) {
+        a.spec_levels = std::min<uint32_t>(b->spec_max, uint32_t(s.p.localsubdiv - 1));
+        a.spec_cap = b->spec_cap;
+        a.spec_list = reinterpret_cast<uint32_t *>(base + L.spec_list);
+        a.spec_count = reinterpret_cast<uint32_t *>(base + L.spec_cnt);
+        a.spec_res = reinterpret_cast<SpecCell *>(base + L.spec_res);
+    }
     s.a = a;
     s.rgba = reinterpret_cast<float4 *>(base + L.rgba);
     s.depth = reinterpret_cast<float *>(base + L.depth);
@@ -326,6 +343,12 @@ int tsi_batch_create(const tsi_gpu_instruction *const *tapes, const uint32_t *le
     if (C.waves_per_simd == 0) C.waves_per_simd = 2;
     if (C.waves_per_simd > 4) C.waves_per_simd = 4;
     if (C.claim_units == 0) C.claim_units = 1;
+    {
+        const char *e = std::getenv("TSI_SPEC_LEVELS");
+        const long v = e ? std::strtol(e, nullptr, 10) : kSpecLevelsDefault;
+        b->spec_max = v <= 0 ? 0u : uint32_t(v < kMaxLevels ? v : kMaxLevels);
+        b->spec_cap = b->spec_max ? (1u << 13) : 0u;   // 8 192 nodes: 4 MiB of records per session
+    }
     int rc = TSI_OK;
     do {
         // ---- tapes: lowering, then the module of all of them ----
@@ -430,7 +453,7 @@ int tsi_batch_footprint(tsi_batch *b, const tsi_session_desc *d, uint64_t *bytes
     uint32_t lc;
     size_t px;
     if (!session_sizes(*d, nodes, lc, px)) return TSI_E_ARG;
-    *bytes = layout(nodes, lc, px).total;
+    *bytes = layout(nodes, lc, px, b->spec_cap).total;
     return TSI_OK;
 }
 
@@ -448,7 +471,7 @@ int tsi_batch_open(tsi_batch *b, const tsi_session_desc *d, uint32_t *session) {
     uint32_t slot = 0;
     while (slot < b->sessions.size() && b->sessions[slot].open) ++slot;
     if (slot == b->sessions.size()) return TSI_E_BUDGET;
-    const Layout L = layout(nodes, lc, px);
+    const Layout L = layout(nodes, lc, px, b->spec_cap);
     uint64_t off = 0;
     if (!arena_take(b, L.total, off)) return TSI_E_BUDGET;
     Session &s = b->sessions[slot];
@@ -517,6 +540,7 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
         return b->sessions[reqs[x].session].tape < b->sessions[reqs[y].session].tape;
     });
     uint32_t njobs = 0;
+    bool any_spec = false;
     for (uint32_t j = 0; j < n; ++j) {
         const uint32_t i = b->order[j];
         const tsi_frame_req &r = reqs[i];
@@ -524,6 +548,8 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
         BatchSession &S = b->sess_h[j];
         std::memset(&S, 0, sizeof S);
         S.a = s.a;
+        S.a.spec_tag = b->spec_tag + 1u ? b->spec_tag + 1u : 1u;
+        any_spec = any_spec || S.a.spec_levels != 0;
         S.time = r.time;
         S.tape = s.tape;
         S.levels = s.p.localsubdiv;
@@ -563,6 +589,8 @@ int tsi_batch_submit(tsi_batch *b, const tsi_frame_req *reqs, uint32_t n) {
     args.stack_len = b->stack_rows;
     args.nsess = n;
     args.phases = levels + 2;
+    args.first_phase = any_spec ? -1 : 0;
+    b->spec_tag = b->spec_tag + 1u ? b->spec_tag + 1u : 1u;
     args.claim_units = b->cfg.claim_units;
     args.wait_ticks = uint64_t(100000000) * 10u;   // 10 s at the 100 MHz real-time counter
     void *kp[] = {&args};

@@ -396,6 +396,47 @@ TSI_DI void eval_level(const KArgs &a, int level, float time, uint32_t wave_, ui
     flush_tally(a.cnt, tl);
 }
 
+// The speculative pass's parents (KArgs::spec_*): every node at depth < spec_levels as the frame
+// starts, breadth first from the root (the 8 children of each node with children), by the nt
+// threads of one block; the list stops at spec_cap entries.
+template <class KA>
+TSI_DI void spec_gather(const KA &a, uint32_t t, uint32_t nt) {
+    __shared__ uint32_t n_sh, begin_sh, end_sh;
+    if (t == 0) {
+        a.spec_list[0] = 0u;
+        n_sh = 1u;
+        begin_sh = 0u;
+        end_sh = 1u;
+    }
+    __syncthreads();
+    constexpr uint32_t K = 4;   // entries per thread per step, their loads issued together
+    for (uint32_t d = 1; d < a.spec_levels; ++d) {   // the nodes at depth d
+        const uint32_t b0 = begin_sh, e0 = end_sh;
+        for (uint32_t i0 = b0 + t; i0 < e0; i0 += K * nt) {
+            uint32_t P[K], m[K];
+            for (uint32_t k = 0; k < K; ++k) {
+                const uint32_t i = i0 + k * nt;
+                P[k] = i < e0 ? a.spec_list[i] : 0u;
+            }
+            for (uint32_t k = 0; k < K; ++k) m[k] = i0 + k * nt < e0 ? meta_of(a.st[P[k]]) & kChildMask : 0u;
+            for (uint32_t k = 0; k < K; ++k) {
+                if (m[k]) {
+                    const uint32_t at = atomicAdd(&n_sh, 8u);
+                    for (uint32_t o = 0; o < 8u; ++o)
+                        if (at + o < a.spec_cap) a.spec_list[at + o] = (m[k] - 1u) * 8u + o;
+                }
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            begin_sh = e0;
+            end_sh = min(n_sh, a.spec_cap);
+        }
+        __syncthreads();
+    }
+    if (t == 0) *a.spec_count = min(n_sh, a.spec_cap);
+}
+
 // ---- the frame's first and last steps (thread t of nt threads) ----
 
 // Start of a frame: clear the per-frame counters and, when buf_main has enqueued it

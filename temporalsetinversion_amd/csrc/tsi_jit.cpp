@@ -915,7 +915,7 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
          "                           LevelTally &tl) {\n"
          "        E eng(a, 0u, lane_id());\n"
          "        Slots Rlo, Rhi;\n"
-         "        eval_group<REV, E>(a, level, time, g, count, stk, eng, Rlo, Rhi, tl);\n"
+         "        eval_group<REV, E>(a, level, time, g, count, stk, eng, Rlo, Rhi, tl, REV && level < 0);\n"
          "    }\n"
          "    template <bool REV, class KA>\n"
          "    static TSI_DI void group(uint32_t tape, const KA &a, int level, float time, uint32_t g, uint32_t count,\n"

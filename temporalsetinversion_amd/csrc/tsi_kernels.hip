@@ -36,46 +36,8 @@ __global__ void __launch_bounds__(256) k_eval(KArgs a, int level, float time) {
 
 __global__ void k_frame_begin(KArgs a) { frame_begin(a, threadIdx.x, blockDim.x); }
 
-// The speculative pass's parents: every node at depth < spec_levels as the frame starts, breadth
-// first from the root (the 8 children of each node with children).  One block; the list stops
-// at spec_cap entries.
-__global__ void __launch_bounds__(1024) k_spec_gather(KArgs a) {
-    __shared__ uint32_t n, begin, end;
-    const uint32_t t = threadIdx.x;
-    if (t == 0) {
-        a.spec_list[0] = 0u;
-        n = 1u;
-        begin = 0u;
-        end = 1u;
-    }
-    __syncthreads();
-    constexpr uint32_t K = 4;   // entries per thread per step, their loads issued together
-    for (uint32_t d = 1; d < a.spec_levels; ++d) {   // the nodes at depth d
-        const uint32_t b0 = begin, e0 = end;
-        for (uint32_t i0 = b0 + t; i0 < e0; i0 += K * blockDim.x) {
-            uint32_t P[K], m[K];
-            for (uint32_t k = 0; k < K; ++k) {
-                const uint32_t i = i0 + k * blockDim.x;
-                P[k] = i < e0 ? a.spec_list[i] : 0u;
-            }
-            for (uint32_t k = 0; k < K; ++k) m[k] = i0 + k * blockDim.x < e0 ? meta_of(a.st[P[k]]) & kChildMask : 0u;
-            for (uint32_t k = 0; k < K; ++k) {
-                if (m[k]) {
-                    const uint32_t at = atomicAdd(&n, 8u);
-                    for (uint32_t o = 0; o < 8u; ++o)
-                        if (at + o < a.spec_cap) a.spec_list[at + o] = (m[k] - 1u) * 8u + o;
-                }
-            }
-        }
-        __syncthreads();
-        if (t == 0) {
-            begin = e0;
-            end = min(n, a.spec_cap);
-        }
-        __syncthreads();
-    }
-    if (t == 0) *a.spec_count = min(n, a.spec_cap);
-}
+// The speculative pass's node list (spec_gather, tsi_eval.h): one block.
+__global__ void __launch_bounds__(1024) k_spec_gather(KArgs a) { spec_gather(a, threadIdx.x, blockDim.x); }
 // every parent decided.  Groups with an entry lacking a record run the full program.
 __global__ void __launch_bounds__(256) k_prune(KArgs a, int level, int which) {
     Counters *cnt = a.cnt;
