@@ -77,6 +77,50 @@ def test_speculative_levels_change_nothing(scene, L, kw):
     assert taken > 0
 
 
+def test_speculative_pass_across_depth_changes_resets_and_frames_in_flight():
+    """The speculative pass runs on a stream of its own beside the previous frame's deep levels:
+    frames queued two at a time, the octree depth changed between frames (tsi_set_params: more
+    or fewer speculative levels than the previous frame ran) and a reset in the middle give every
+    frame the nodes and statistics of the same frames without speculation."""
+    tape = tape_of("capsules")
+    plan = [(0, 6, False), (1, 6, False), (2, 7, False), (3, 5, False), (4, 5, True), (5, 7, False),
+            (6, 7, False), (7, 6, False)]   # (frame, localsubdiv, reset before it)
+    runs = {}
+    for spec in (True, False):
+        out = []
+        with T.State(tape, localsubdiv=6, voxel_capacity=1 << 20, prune=False, lipschitz_corrected=True,
+                     speculate=spec) as st:
+            st.reset()
+            pending = []
+            for k, L, reset in plan:
+                if reset or L != st.params.localsubdiv:
+                    while pending:
+                        out.append((pending.pop(0), st.set_inversion_result().stats, None))
+                    if reset:
+                        st.reset()
+                    if L != st.params.localsubdiv:
+                        st.set_params(localsubdiv=L)
+                st.update(float(np.float32(k / 60.0)))
+                st.set_inversion_async()
+                st.enqueue_root()
+                pending.append(k)
+                if len(pending) == 2:
+                    out.append((pending.pop(0), st.set_inversion_result().stats, None))
+            while pending:
+                out.append((pending.pop(0), st.set_inversion_result().stats, None))
+            out.append((-1, None, st.read_nodes()))
+        runs[spec] = out
+    for (k1, s1, n1), (k0, s0, n0) in zip(runs[True], runs[False]):
+        assert k1 == k0
+        if s1 is not None:
+            print(f"frame {k1}: cells {s1['cell_evals']}, from the speculative pass {s1['spec_cells']}", flush=True)
+            for key in ("cell_evals", "rev_evals", "fwd_ops", "rev_ops", "parents", "enqueued", "deallocs"):
+                assert s1[key] == s0[key], f"frame {k1}: {key}"
+        else:
+            assert_same_nodes(n1, n0, "after the sequence")
+    assert sum(s["spec_cells"] for _, s, _ in runs[True] if s is not None) > 0
+
+
 @pytest.mark.parametrize("engine", ["compiled-full", "interp-full"])
 def test_stats_match_oracle(engine):
     tape = tape_of("spheres")
