@@ -213,6 +213,7 @@ def summarize_profile(args) -> None:
                 lines.append(f"{k:<24} {'hbm':>4} {len(vf):>10} {sum(vf) / len(vf):>15.1f} {sum(vw) / len(vw):>15.1f} "
                              f"{b:>15.0f}")
     for path in args.pmc_sq:
+        _valu_issue_per_level(path, L, lines)
         rows = _csv_rows(path)
         counters = sorted({r["Counter_Name"] for r in rows})
         vals = {}
@@ -229,6 +230,50 @@ def summarize_profile(args) -> None:
             if v:
                 lines.append(f"  {c:<28} {sum(v) / len(v):>16.1f}")
     print("\n".join(lines))
+
+
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles at 2.4 GHz
+VALU_ISSUE_PER_US = 256 * 4 * 2400 / 4
+
+
+def _valu_issue_per_level(path, L, lines):
+    """SQ_INSTS_VALU per level against the issue capacity of the same dispatch's duration (counter
+    run, frames after the first): how close each level runs to one VALU instruction per SIMD
+    every 4 cycles."""
+    rows = sorted(_csv_rows(path), key=lambda r: int(r["Dispatch_Id"]))
+    if not any(r["Counter_Name"] == "SQ_INSTS_VALU" for r in rows):
+        return
+    frame, spec_q, seen, per, pos = -1, set(), set(), {}, {}
+    for r in rows:
+        k, q, i = _kernel(r["Kernel_Name"]), r["Queue_Id"], int(r["Dispatch_Id"])
+        if i in seen:
+            continue
+        seen.add(i)
+        if k == "tsi::k_frame_begin":
+            frame += 1
+            pos[frame] = 0
+        if k == "tsi::k_spec_gather":
+            spec_q.add(q)
+        if k != "tsi_tape_eval_level" or frame < 1:
+            continue
+        v = [float(x["Counter_Value"]) for x in rows if int(x["Dispatch_Id"]) == i and x["Counter_Name"] == "SQ_INSTS_VALU"]
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        if q in spec_q:
+            lab = "spec"
+        else:
+            lab = pos[frame]
+            pos[frame] += 1
+        per.setdefault(lab, []).append((v[0] if v else 0.0, us))
+    lines += ["", "VALU issue per level (counter run, frames after the first): SQ_INSTS_VALU per dispatch and its share "
+              "of the issue", f"capacity of the dispatch's duration ({VALU_ISSUE_PER_US / 1e6:.2f} M wave64 instructions "
+              "per us: 1024 SIMDs, one every 4 cycles, 2.4 GHz)"]
+    for lab in sorted(per, key=lambda x: -1 if x == "spec" else x):
+        if lab != "spec" and lab >= L:
+            continue
+        insts = sum(x for x, _ in per[lab]) / len(per[lab])
+        us = sum(u for _, u in per[lab]) / len(per[lab])
+        lines.append(f"  level {lab:>4} {insts / 1e6:>10.2f} M instructions in {us:>8.1f} us  "
+                     f"{100 * insts / (VALU_ISSUE_PER_US * us):>6.1f} %")
 
 
 def summarize_kernels(args) -> None:
