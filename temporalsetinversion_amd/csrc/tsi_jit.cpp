@@ -324,6 +324,12 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // then saved once.
     std::vector<int> slot(n, -1), cidx(n, -1), remat(n, -1);
     uint32_t nslots = 0, ncodes = 0;
+    // SQRT records whose reverse rule takes the record's own value (sqrt_rr_of_value) instead of
+    // SQRT_I of the saved operand: where fix is the identity, the saved value is that square
+    // root, bit for bit (unpruned modules)
+    std::vector<char> sqv(n, 0);
+    for (uint32_t j = 4; j < n; ++j)
+        sqv[j] = lean && ff[j] && fwd[j - 4].form == F_SQRT && side_of(tape[j]) == 1;
     auto remat_form = [&](uint32_t k) {
         const uint32_t f = fwd[k - 4].form;
         return k >= 4 && (f == F_ADD_RL || f == F_SUB_RL || f == F_MUL_RL || f == F_NEG || f == F_SUB_LR);
@@ -338,6 +344,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (is_code_form(form)) { cidx[j] = int(ncodes++); continue; }
             bool nx = false, ny = false;
             rev_needs(form, nx, ny);
+            if (sqv[j]) { need[j] = 1; continue; }
             if (nx) need[side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])] = 1;
             if (ny) need[rarg_of(tape[j])] = 1;
         }
@@ -479,6 +486,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const uint32_t side = side_of(tape[j]);
             bool nx = false, ny = false;
             rev_needs(fwd[j - 4].form, nx, ny);
+            if (sqv[j]) return second ? -1 : int(j);   // the record's own value
             if (!second) return nx ? int(side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])) : -1;
             return ny ? int(rarg_of(tape[j])) : -1;
         };
@@ -611,6 +619,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     const bool mm = pr && mmx[j] >= 0;
                     // the rule's derivatives: from the saved values, or from the decision code
                     std::string rule = "rev_form_t<true>(" + form + ", " + x + ", " + y + ", " + lit(g.literal) + ")";
+                    if (sqv[j]) rule = "Iv2{sqrt_rr_of_value(" + x + "), ZI}";
                     if (cidx[j] >= 0) {
                         const uint32_t w = uint32_t(cidx[j]) / 16u;
                         o += "        const uint32_t kc" + J + " = (kr" + std::to_string(w) + " >> " +

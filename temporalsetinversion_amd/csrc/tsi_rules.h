@@ -109,6 +109,16 @@ TSI_DI Iv fwd_form_t(uint32_t form, Iv l, Iv r, float c) {
     }
 }
 
+// SQRT_I_RR (compute.wgsl:2878-2882) given v = SQRT_I(x): 1 / (2 v), the upper bound capped at
+// 1e6.  The compiled engine passes the forward value of the record itself where its fix is the
+// identity (then the same bits as SQRT_I of the saved operand), so the reverse pass runs no
+// square root.
+TSI_DI Iv sqrt_rr_of_value(Iv v) {
+    Iv d = DIV_II(I_FROM(1.0f), MUL_IF(v, 2.0f));
+    d.hi = wmin(d.hi, kBig);
+    return d;
+}
+
 // Reverse rule of a form, before fix (compute.wgsl:2806-2970 as called from 3786-3963): the
 // derivative with respect to the first register operand in .a (`x` the saved value of it) and
 // to the second in .b (`y`, two-register forms).  MIN/MAX give the undecided choice rules
@@ -120,10 +130,7 @@ TSI_DI Iv2 rev_form_t(uint32_t form, Iv x, Iv y, float c) {
     switch (form) {
     // one register operand (side 1)
     case F_SQR: d.a = MUL_IF(x, 2.0f); break;
-    case F_SQRT:
-        d.a = DIV_II(ONE, MUL_IF(SQRT_I(x), 2.0f));
-        d.a.hi = wmin(d.a.hi, kBig);
-        break;
+    case F_SQRT: d.a = sqrt_rr_of_value(SQRT_I(x)); break;
     case F_NEG: d.a = I_FROM(-1.0f); break;
     case F_ADD_RL: case F_SUB_RL: case F_RET: case F_MOD_RL: d.a = ONE; break;
     case F_MUL_RL: d.a = I_FROM(c); break;
