@@ -29,10 +29,11 @@ std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const st
 // (rev_kernel) forward with saved operands, then reverse; with a kernel of its own
 // (tsi_tape_eval_fwd / tsi_tape_eval_rev), which jit_source_level replaces by one for both.  prune: regions dead for
 // all of a wavefront's parents are skipped and decided MIN/MAX become copies (liveness records
-// written for enqueued cells).
+// written for enqueued cells).  skip: the reverse pass of an unpruned module skips subtrees whose
+// sensitivity is [0, 0] in the whole wavefront.
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
                        const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
-                       const JitDomain *domain = nullptr);
+                       const JitDomain *domain = nullptr, bool skip = true);
 // Both level kernels' code in one module with one kernel, tsi_tape_eval_level(KArgs, level,
 // time), that runs a level's forward-only and reverse lists side by side (blockDim 256).
 std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
