@@ -98,6 +98,34 @@ uint32_t level_waves() {
     return w;
 }
 
+// Largest cost (in estimated VALU instructions per interval) of the forward code that the reverse
+// pass of an unpruned module reruns to recompute a value instead of saving it
+// ($TSI_JIT_REMAT, default 160; 0 keeps the one-step recomputation of literal forms only).  On the
+// 1024^3 CSG frame (343 saved values without it), budgets of 80, 160 and 320 save 168, 74 and
+// fewer and gave 334, 355 and 353 M cell-evaluations/s (305 without).
+uint32_t remat_budget() {
+    static const uint32_t b = [] {
+        const char *e = std::getenv("TSI_JIT_REMAT");
+        const long v = e ? std::strtol(e, nullptr, 10) : 160;
+        return v >= 0 && v <= 100000 ? uint32_t(v) : 160u;
+    }();
+    return b;
+}
+
+// Estimated VALU instructions of a form's forward value (0: never recomputed: the transcendental
+// and division forms, whose code is long).
+uint32_t recompute_cost(uint32_t form) {
+    switch (form) {
+    case F_ADD_RR: case F_SUB_RR: case F_ADD_RL: case F_SUB_RL: case F_SUB_LR: case F_NEG: case F_MUL_RL: return 2;
+    case F_MIN_RR: case F_MAX_RR: case F_MIN_RL: case F_MAX_RL: return 4;
+    case F_ABS: return 6;
+    case F_SQR: return 8;
+    case F_MUL_RR: return 12;
+    case F_SQRT: return 36;
+    default: return 0;
+    }
+}
+
 // Smallest subtree (records dominated by one record) that the reverse pass of an unpruned module
 // skips when the dominating record's sensitivity is [0, 0] in every lane of the wavefront
 // ($TSI_JIT_SKIP, default 8; 0 turns the skipping off).  Each guard costs a uniform test and a
@@ -337,6 +365,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // binary32 operations give the same bits, and a coordinate shared by many primitives is
     // then saved once.
     std::vector<int> slot(n, -1), cidx(n, -1), remat(n, -1);
+    std::vector<char> recomp(n, 0);   // recomputed from its operands (unpruned modules)
     uint32_t nslots = 0, ncodes = 0;
     // SQRT records whose reverse rule takes the record's own value (sqrt_rr_of_value) instead of
     // SQRT_I of the saved operand: where fix is the identity, the saved value is that square
@@ -362,18 +391,52 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (nx) need[side == 2 ? rarg_of(tape[j]) : larg_of(tape[j])] = 1;
             if (ny) need[rarg_of(tape[j])] = 1;
         }
-        std::vector<uint32_t> sharing(n, 0);
-        for (uint32_t k = 4; k < n; ++k)
-            if (need[k] && remat_form(k)) sharing[reg_operand(k)] += 1;
-        for (uint32_t k = 4; k < n; ++k) {   // ascending: an operand is final when reached
-            if (!need[k] || !remat_form(k)) continue;
-            // the operand (a < k) is decided already: saved, recomputed, or not needed so far
-            const uint32_t a = reg_operand(k);
-            const bool a_saved = need[a] && remat[a] < 0;
-            if (remat[a] < 0 && (a_saved || sharing[a] >= 2)) {
-                remat[k] = int(a);
+        if (lean && remat_budget() > 0) {
+            // Unpruned modules: a needed value whose forward code, down to values that are saved
+            // (the inputs, values of 4 or more users such as a coordinate shared by many
+            // primitives, and values too costly to recompute), costs at most remat_budget() is
+            // recomputed by the reverse pass with the same forward expression (the same binary32
+            // operations, the same bits) instead of saved: the saved-operand stack is written by
+            // every cell and read back by every cell, while the recomputation runs only where the
+            // record's rule runs (not in a skipped subtree).
+            std::vector<uint32_t> nusers(n, 0), rc(n, 0);
+            std::vector<char> anchor(n, 1);
+            for (uint32_t j = 4; j < n; ++j) {
+                const uint32_t side = side_of(tape[j]);
+                if (side == 0 || side == 1) nusers[larg_of(tape[j])] += 1;
+                if (side == 0 || side == 2) nusers[rarg_of(tape[j])] += 1;
+            }
+            for (uint32_t k = 4; k < n; ++k) {   // ascending: the operands' costs are known
+                const uint32_t side = side_of(tape[k]), ic = recompute_cost(fwd[k - 4].form);
+                if (side == 3 || ic == 0) continue;
+                uint32_t c = ic + (ff[k] ? 0u : 4u);
+                if (side == 0 || side == 1) c += anchor[larg_of(tape[k])] ? 0u : rc[larg_of(tape[k])];
+                if (side == 0 || side == 2) c += anchor[rarg_of(tape[k])] ? 0u : rc[rarg_of(tape[k])];
+                rc[k] = c;
+                anchor[k] = nusers[k] >= 4u || c > remat_budget();
+            }
+            for (uint32_t k = n; k-- > 4;) {   // descending: a recomputed value's operands become needed
+                if (!need[k] || anchor[k]) continue;
+                recomp[k] = 1;
                 need[k] = 0;
-                need[a] = 1;
+                const uint32_t side = side_of(tape[k]);
+                if (side == 0 || side == 1) need[larg_of(tape[k])] = 1;
+                if (side == 0 || side == 2) need[rarg_of(tape[k])] = 1;
+            }
+        } else {
+            std::vector<uint32_t> sharing(n, 0);
+            for (uint32_t k = 4; k < n; ++k)
+                if (need[k] && remat_form(k)) sharing[reg_operand(k)] += 1;
+            for (uint32_t k = 4; k < n; ++k) {   // ascending: an operand is final when reached
+                if (!need[k] || !remat_form(k)) continue;
+                // the operand (a < k) is decided already: saved, recomputed, or not needed so far
+                const uint32_t a = reg_operand(k);
+                const bool a_saved = need[a] && remat[a] < 0;
+                if (remat[a] < 0 && (a_saved || sharing[a] >= 2)) {
+                    remat[k] = int(a);
+                    need[k] = 0;
+                    need[a] = 1;
+                }
             }
         }
         for (uint32_t k = 0; k < n; ++k)
@@ -591,7 +654,13 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         std::function<std::string(uint32_t, int)> saved = [&](uint32_t j, int k) -> std::string {
             if (k < 0) return "ZI";
             std::string v;
-            if (remat[k] >= 0) {   // recomputed from its saved register operand
+            if (recomp[k]) {   // recomputed: the forward expression over recomputed or saved operands
+                const uint32_t side = side_of(tape[k]);
+                const std::string l = (side == 0 || side == 1) ? saved(j, int(larg_of(tape[k]))) : "ZI";
+                const std::string r = (side == 0 || side == 2) ? saved(j, int(rarg_of(tape[k]))) : "ZI";
+                v = fixed(uint32_t(k), "fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " + l + ", " + r +
+                                           ", " + lit(tape[k].literal) + ")");
+            } else if (remat[k] >= 0) {   // recomputed from its saved register operand
                 const std::string x = saved(j, remat[k]);
                 const bool lr = side_of(tape[k]) == 2;
                 v = fixed(uint32_t(k), "fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " +
@@ -602,6 +671,13 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (pr && k >= 4 && region_of(uint32_t(k)) != region_of(j))
                 return "(" + bit(region_of(uint32_t(k))) + " ? " + v + " : ZI)";
             return v;
+        };
+        // the saved values a value is read or recomputed from
+        std::function<void(uint32_t, std::vector<char> &)> leaves = [&](uint32_t k, std::vector<char> &out) {
+            if (!recomp[k]) { out[k] = 1; return; }
+            const uint32_t side = side_of(tape[k]);
+            if (side == 0 || side == 1) leaves(larg_of(tape[k]), out);
+            if (side == 0 || side == 2) leaves(rarg_of(tape[k]), out);
         };
         // saved values (sv) and code words (kr) read by records hi-1 .. lo
         auto region_needs = [&](uint32_t hi, uint32_t lo, std::vector<char> &vneed, std::vector<char> &wneed) {
@@ -616,7 +692,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 for (bool second : {false, true}) {
                     int k = operand_of(jj, second);
                     if (k >= 0 && remat[k] >= 0) k = remat[k];
-                    if (k >= 0) vneed[k] = 1;
+                    if (k >= 0) leaves(uint32_t(k), vneed);
                 }
             }
         };
