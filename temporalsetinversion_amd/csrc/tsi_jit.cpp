@@ -126,6 +126,15 @@ uint32_t recompute_cost(uint32_t form) {
     }
 }
 
+// Batch modules with the skipped subtrees ($TSI_JIT_BATCH_SKIP=1; default without).
+bool batch_skip() {
+    static const bool b = [] {
+        const char *e = std::getenv("TSI_JIT_BATCH_SKIP");
+        return e && std::strtol(e, nullptr, 10) != 0;
+    }();
+    return b;
+}
+
 // Smallest subtree (records dominated by one record) that the reverse pass of an unpruned module
 // skips when the dominating record's sensitivity is [0, 0] in every lane of the wavefront
 // ($TSI_JIT_SKIP, default 8; 0 turns the skipping off).  Each guard costs a uniform test and a
@@ -443,12 +452,19 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (need[k]) slot[k] = int(nslots++);
     }
     const uint32_t code_words = (ncodes + 15u) / 16u;
-    // the region after which code word w is complete
+    // Unpruned modules save the code words two to a row (word w in row nslots + w / 2, .x for
+    // even w), a row when its second word (or the last word) is complete; pruned modules one to a
+    // row (.x).
+    const uint32_t code_rows = lean ? (code_words + 1u) / 2u : code_words;
+    // the region after which code word w is complete (lean: the rows, after their last word)
     std::vector<std::vector<uint32_t>> flush_after(Y.regions + 1);
     for (uint32_t j = 4; j < n; ++j)
-        if (cidx[j] >= 0 && (cidx[j] % 16 == 15 || uint32_t(cidx[j]) == ncodes - 1u))
-            flush_after[region_of(j)].push_back(uint32_t(cidx[j]) / 16u);
-    if (rev_kernel && nslots + code_words > L.stack + 1u)
+        if (cidx[j] >= 0 && (cidx[j] % 16 == 15 || uint32_t(cidx[j]) == ncodes - 1u)) {
+            const uint32_t w = uint32_t(cidx[j]) / 16u;
+            if (!lean) flush_after[region_of(j)].push_back(w);
+            else if (w % 2u == 1u || w + 1u == code_words) flush_after[region_of(j)].push_back(w / 2u);
+        }
+    if (rev_kernel && nslots + code_rows > L.stack + 1u)
         throw std::runtime_error("saved-operand stack layout exceeds the allocation");
     auto code_expr = [&](uint32_t j, const std::string &l, const std::string &rhs) {
         const uint32_t form = fwd[j - 4].form;
@@ -523,9 +539,17 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
         }
         if ((j - 4) % G == G - 1 || j == n - 1) {
             o += close_region();
-            for (uint32_t w : flush_after[region_of(j)])
-                o += "    if (PUSH) " + P(nslots + w) + " = make_float2(__uint_as_float(kw" + std::to_string(w) +
-                     "), 0.0f);\n";
+            for (uint32_t w : flush_after[region_of(j)]) {
+                if (!lean) {
+                    o += "    if (PUSH) " + P(nslots + w) + " = make_float2(__uint_as_float(kw" + std::to_string(w) +
+                         "), 0.0f);\n";
+                    continue;
+                }
+                const std::string hi_word = 2u * w + 1u < code_words ? "__uint_as_float(kw" + std::to_string(2u * w + 1u) + ")"
+                                                                      : std::string("0.0f");
+                o += "    if (PUSH) " + P(nslots + w) + " = make_float2(__uint_as_float(kw" + std::to_string(2u * w) +
+                     "), " + hi_word + ");\n";
+            }
         }
     }
     if (choices)
@@ -672,6 +696,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 return "(" + bit(region_of(uint32_t(k))) + " ? " + v + " : ZI)";
             return v;
         };
+        uint32_t nrow_loads = 0;   // loads of code-word rows (each its own temporary)
         // the saved values a value is read or recomputed from
         std::function<void(uint32_t, std::vector<char> &)> leaves = [&](uint32_t k, std::vector<char> &out) {
             if (!recomp[k]) { out[k] = 1; return; }
@@ -723,9 +748,17 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             }
             std::vector<char> vneed, wneed;
             region_needs(hi, lo, vneed, wneed);
-            for (uint32_t w = 0; w <= code_words; ++w)
-                if (wneed[w] && !(wskip && (*wskip)[w]))
-                    o += "        kr" + std::to_string(w) + " = __float_as_uint(" + P(nslots + w) + ".x);\n";
+            // a row's two words with one load; a word the regions running before its use read
+            // too (skip) is in a register already and is not assigned again
+            for (uint32_t m = 0; 2u * m < code_words; ++m) {
+                const bool a = wneed[2u * m] && !(wskip && (*wskip)[2u * m]);
+                const bool b = 2u * m + 1u < code_words && wneed[2u * m + 1u] && !(wskip && (*wskip)[2u * m + 1u]);
+                if (!a && !b) continue;
+                const std::string t = "crow" + std::to_string(nrow_loads++);
+                o += "        const float2 " + t + " = " + P(nslots + m) + ";\n";
+                if (a) o += "        kr" + std::to_string(2u * m) + " = __float_as_uint(" + t + ".x);\n";
+                if (b) o += "        kr" + std::to_string(2u * m + 1u) + " = __float_as_uint(" + t + ".y);\n";
+            }
             for (uint32_t k = n; k-- > 0;)
                 if (vneed[k] && !(vskip && (*vskip)[k]))
                     o += "        sv" + std::to_string(k) + " = " + P(uint32_t(slot[k])) + ";\n";
@@ -1107,8 +1140,9 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
         const BatchTape &t = tapes[i];
         // without the skipped subtrees: the batched workload's wide swept windows leave most
         // MIN/MAX undecided, and its 256 requests ran 6 % slower with them
-        const std::string f = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, false, false, domain, false);
-        const std::string r = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, true, false, domain, false);
+        const bool skip = batch_skip();
+        const std::string f = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, false, false, domain, skip);
+        const std::string r = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, true, false, domain, skip);
         o += "namespace tb" + std::to_string(i) + " {\nnamespace jf {   // forward-only list\n" + module_body(f);
         o += "}  // namespace jf\nnamespace jr {   // reverse list\n" + module_body(r);
         o += "}  // namespace jr\n}  // namespace tb" + std::to_string(i) + "\n";
