@@ -232,7 +232,25 @@ TSI_DI Iv MUL_IF(Iv a, float b) {                                               
     const float x = a.lo * b, y = a.hi * b;
     return b >= 0.0f ? I(x, y) : I(y, x);
 }
-TSI_DI Iv SQRT_I(Iv a) { return I(sqrtf(wmax(a.lo, 0.0f)), sqrtf(wmax(a.hi, 0.0f))); }  // 2572
+// Correctly rounded square root of x in [-0, +inf] (SQRT_I's operands: wmax(v, 0) maps NaN and
+// negative values to +0).  The method of LLVM's AMDGPU lowering of a correctly rounded binary32
+// sqrt: inputs below 2^-96 scaled by 2^32 (exact), the hardware square root s (within 1 ulp), the
+// neighbour s - 1 ulp taken when x - (s - 1 ulp) s <= 0 and s + 1 ulp when x - (s + 1 ulp) s > 0
+// (exact residuals by fma), the result scaled back by 2^-16.  The lowering's last step, which
+// returns x itself for +-0 and +inf, is left out: for x = +-0 and +inf both residual tests fail
+// (a NaN neighbour or residual, or a zero residual) and s, which is then x, is returned.  Equal,
+// bit for bit, to sqrtf for every binary32 in [-0, +inf] (an exhaustive GPU test).
+TSI_DI float sqrt_nonneg(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-sd, s, xs) <= 0.0f ? sd : s;
+    r = __builtin_fmaf(-su, s, xs) > 0.0f ? su : r;
+    return tiny ? r * 0x1p-16f : r;
+}
+TSI_DI Iv SQRT_I(Iv a) { return I(sqrt_nonneg(wmax(a.lo, 0.0f)), sqrt_nonneg(wmax(a.hi, 0.0f))); }  // 2572
 TSI_DI Iv EXP_I(Iv a) { return I(det_exp(a.lo), det_exp(a.hi)); }                      // 2576
 TSI_DI Iv LOG_I(Iv a) { return I(det_log(wmax(a.lo, 0.0f)), det_log(wmax(a.hi, 0.0f))); }  // 2584
 TSI_DI Iv DIV_II(Iv a, Iv b) {                                                           // 2598

@@ -452,6 +452,14 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             if (need[k]) slot[k] = int(nslots++);
     }
     const uint32_t code_words = (ncodes + 15u) / 16u;
+    // bit position of code c in its word: pruned modules place code i of a word at bits 2i;
+    // unpruned ones shift each word left by 2 as its codes are appended, so the word's last code
+    // sits at bits 0-1
+    auto code_shift = [&](uint32_t c) -> uint32_t {
+        if (!lean) return 2u * (c % 16u);
+        const uint32_t in_word = std::min(16u, ncodes - 16u * (c / 16u));
+        return 2u * (in_word - 1u - c % 16u);
+    };
     // Unpruned modules save the code words two to a row (word w in row nslots + w / 2, .x for
     // even w), a row when its second word (or the last word) is complete; pruned modules one to a
     // row (.x).
@@ -534,8 +542,12 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             // a decided MIN/MAX of the pruned variant keeps the parent's choice (its code is
             // not read: the reverse pass takes the mode)
             const std::string code = (pr && mmx[j] >= 0) ? "ch" + std::to_string(j) : code_expr(j, l, rhs);
-            o += "        if (PUSH) kw" + std::to_string(cidx[j] / 16) + " |= " + code + " << " +
-                 std::to_string(2 * (cidx[j] % 16)) + "u;\n";
+            if (lean)   // appended: the constants stay inline (no shifted select constants to materialise)
+                o += "        if (PUSH) kw" + std::to_string(cidx[j] / 16) + " = (kw" + std::to_string(cidx[j] / 16) +
+                     " << 2u) | " + code + ";\n";
+            else
+                o += "        if (PUSH) kw" + std::to_string(cidx[j] / 16) + " |= " + code + " << " +
+                     std::to_string(2 * (cidx[j] % 16)) + "u;\n";
         }
         if ((j - 4) % G == G - 1 || j == n - 1) {
             o += close_region();
@@ -858,7 +870,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                     if (cidx[j] >= 0) {
                         const uint32_t w = uint32_t(cidx[j]) / 16u;
                         o += "        const uint32_t kc" + J + " = (kr" + std::to_string(w) + " >> " +
-                             std::to_string(2 * (cidx[j] % 16)) + "u) & 3u;\n";
+                             std::to_string(code_shift(uint32_t(cidx[j]))) + "u) & 3u;\n";
                         const std::string kc = "kc" + J;
                         if (f.form == F_ABS)
                             rule = "Iv2{sel(" + kc + " == 3u, I(-1.0f, 1.0f), sel(" + kc + " == 2u, I_FROM(-1.0f), ONE)), ZI}";

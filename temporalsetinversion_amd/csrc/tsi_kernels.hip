@@ -788,3 +788,55 @@ int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s) {
 }
 
 }  // namespace tsi
+
+// ---- check of the interval square root against the compiler's correctly rounded sqrtf ----
+
+namespace tsi {
+
+__global__ void __launch_bounds__(256) k_check_sqrt(uint32_t first, uint32_t count, unsigned long long *bad,
+                                                    uint32_t *first_bad, uint32_t *done) {
+    const uint32_t stride = gridDim.x * blockDim.x;   // count <= 2^31: i + stride never wraps
+    uint32_t n = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride, ++n) {
+        const uint32_t u = first + i;
+        const float x = __uint_as_float(u);
+        if (__float_as_uint(sqrt_nonneg(x)) != __float_as_uint(sqrtf(x))) {
+            atomicAdd(bad, 1ull);
+            atomicMin(first_bad, u);
+        }
+    }
+    if (n) atomicAdd(done, n);
+}
+
+}  // namespace tsi
+
+// Diagnostic export (no counterpart in the reference; not part of the tsi.h interface): compares
+// sqrt_nonneg, the square root of SQRT_I, with sqrtf bit for bit for the binary32 values with bit
+// patterns first .. first + count - 1 on the current device.  *mismatches receives the number of
+// differences, *first_bad the smallest differing pattern (0xFFFFFFFF: none), *checked the number
+// of values compared (count, when every thread ran).
+extern "C" int tsi_check_sqrt(uint32_t first, uint32_t count, uint64_t *mismatches, uint32_t *first_bad,
+                              uint32_t *checked) {
+    if (!mismatches || !first_bad || !checked || count == 0 || count > 0x80000000u ||
+        uint64_t(first) + count > (1ull << 32))
+        return TSI_E_ARG;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, 16) != hipSuccess) return TSI_E_OOM;
+    unsigned long long *bad = static_cast<unsigned long long *>(buf);
+    uint32_t *fb = reinterpret_cast<uint32_t *>(static_cast<char *>(buf) + 8);
+    const uint32_t init[4] = {0u, 0u, 0xFFFFFFFFu, 0u};
+    int rc = TSI_OK;
+    if (hipMemcpy(buf, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) rc = TSI_E_HIP;
+    if (rc == TSI_OK) {
+        hipLaunchKernelGGL(tsi::k_check_sqrt, dim3(4096), dim3(256), 0, 0, first, count, bad, fb, fb + 1);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = TSI_E_HIP;
+    }
+    uint32_t out[4] = {0u, 0u, 0u, 0u};
+    if (rc == TSI_OK && hipMemcpy(out, buf, sizeof out, hipMemcpyDeviceToHost) != hipSuccess) rc = TSI_E_HIP;
+    hipFree(buf);
+    if (rc != TSI_OK) return rc;
+    *mismatches = uint64_t(out[0]) | (uint64_t(out[1]) << 32);
+    *first_bad = out[2];
+    *checked = out[3];
+    return TSI_OK;
+}
