@@ -88,6 +88,9 @@ struct tsi_ctx {
     uint32_t *spec_cnt = nullptr;
     SpecCell *spec_res = nullptr;
     uint32_t spec_cap = 0, spec_tag = 0, spec_max = 0;
+    // events around each level's launch (tsi_stats.eval_kernel_ms as the levels' summed kernel
+    // time); $TSI_LEVEL_EVENTS=0 leaves them out (eval_kernel_ms is then the levels' span)
+    bool level_events = true;
     // the pass runs on its own stream with its own saved-operand stack (one block per CU), after
     // the previous frame's levels up to spec_max have run (spec_ready, recorded by that frame)
     hipStream_t spec_stream = nullptr;
@@ -498,9 +501,14 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         bool evok = true;
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         for (auto &f : c->fslot) {
-            for (hipEvent_t *e : {&f.begin, &f.end, &f.eval0, &f.eval1, &f.spec0, &f.spec1})
+            for (hipEvent_t *e : {&f.begin, &f.end, &f.spec0, &f.spec1})
                 evok = evok && hipEventCreate(e) == hipSuccess;
-            for (hipEvent_t &e : f.level) evok = evok && hipEventCreate(&e) == hipSuccess;
+            // events that only time the levels (nothing waits on them): without the system-scope
+            // fence, whose cache writeback and invalidation between two levels cost ~2 % of a frame
+            for (hipEvent_t *e : {&f.eval0, &f.eval1})
+                evok = evok && hipEventCreateWithFlags(e, hipEventDisableSystemFence) == hipSuccess;
+            for (hipEvent_t &e : f.level)
+                evok = evok && hipEventCreateWithFlags(&e, hipEventDisableSystemFence) == hipSuccess;
             evok = evok && hipHostMalloc((void **)&f.h, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
         }
         for (auto &t : c->tslot) {
@@ -525,6 +533,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             for (auto &x : l)
                 if (!rc) rc = dmalloc(&x, c->list_cap);
         if (rc) break;
+        if (const char *e = std::getenv("TSI_LEVEL_EVENTS")) c->level_events = std::strtol(e, nullptr, 10) != 0;
         // speculative pass: up to 2^16 listed parents (8 records of 64 B each: 32 MiB)
         c->spec_cap = 1u << 16;
         {
@@ -728,7 +737,7 @@ int tsi_set_inversion_async(tsi_ctx *c) {
     FrameLaunch fl{c->p.localsubdiv, c->u.time, c->grid_fwd, c->grid_rev, c->grid_misc,
                    (void *)c->eval_fn[0], f.spec ? &sl : nullptr};
     HIP_OK(hipEventRecord(f.begin, c->stream));
-    if (launch_frame(a, fl, c->stream, f.eval0, f.eval1, f.level) != 0) return TSI_E_HIP;
+    if (launch_frame(a, fl, c->stream, f.eval0, f.eval1, c->level_events ? f.level : nullptr) != 0) return TSI_E_HIP;
     HIP_OK(hipMemcpyAsync(f.h, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipEventRecord(f.end, c->stream));
     c->f_issued += 1;
@@ -750,8 +759,8 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->gpu_ms = ms;
         stats->eval_ms = ems;
-        double kms = 0.0;
-        for (int d = 0; d < c->p.localsubdiv; ++d) {
+        double kms = c->level_events ? 0.0 : double(ems);
+        for (int d = 0; c->level_events && d < c->p.localsubdiv; ++d) {
             float x = 0.0f;
             hipEventElapsedTime(&x, f.level[2 * d], f.level[2 * d + 1]);
             kms += x;
