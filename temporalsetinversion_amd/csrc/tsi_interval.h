@@ -239,7 +239,7 @@ TSI_DI Iv MUL_IF(Iv a, float b) {                                               
 // (exact residuals by fma), the result scaled back by 2^-16.  The lowering's last step, which
 // returns x itself for +-0 and +inf, is left out: for x = +-0 and +inf both residual tests fail
 // (a NaN neighbour or residual, or a zero residual) and s, which is then x, is returned.  Equal,
-// bit for bit, to sqrtf for every binary32 in [-0, +inf] (an exhaustive GPU test).
+// bit for bit, to sqrtf for every binary32 in [-0, +inf] and every NaN (an exhaustive GPU test).
 TSI_DI float sqrt_nonneg(float x) {
     const bool tiny = x < 0x1p-96f;
     const float xs = tiny ? x * 0x1p32f : x;

@@ -126,6 +126,18 @@ uint32_t recompute_cost(uint32_t form) {
     }
 }
 
+// Wavefront priority of the level kernel's frame levels (s_setprio; $TSI_JIT_PRIO, 0-3, default
+// 3): the previous frame's trace runs beside them on the same SIMDs and is not on the frame's
+// critical path.  On the 1024^3 CSG frame: 376-377 M cell-evaluations/s at 0, 382 at 3.
+uint32_t level_prio() {
+    static const uint32_t p = [] {
+        const char *e = std::getenv("TSI_JIT_PRIO");
+        const long v = e ? std::strtol(e, nullptr, 10) : 3;
+        return v >= 0 && v <= 3 ? uint32_t(v) : 3u;
+    }();
+    return p;
+}
+
 // Batch modules with the skipped subtrees ($TSI_JIT_BATCH_SKIP=1; default without).
 bool batch_skip() {
     static const bool b = [] {
@@ -1131,8 +1143,11 @@ std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const 
     // one launch per level: the forward-only list and the reverse list run side by side, each
     // on its share of the grid (level_split), instead of one after the other
     const std::string lb = level_waves() ? "256, " + std::to_string(level_waves()) : "256";
-    o += "extern \"C\" __global__ void __launch_bounds__(" + lb + ") tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n"
-         "    const uint32_t nb = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6;\n"
+    o += "extern \"C\" __global__ void __launch_bounds__(" + lb + ") tsi_tape_eval_level(tsi::KArgs a, int level, float time) {\n";
+    if (level_prio())
+        o += "    // issue ahead of other kernels' wavefronts on the same SIMD (the previous frame's trace)\n"
+             "    if (level >= 0) __builtin_amdgcn_s_setprio(" + std::to_string(level_prio()) + ");\n";
+    o += "    const uint32_t nb = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6;\n"
          "    // level -1: the frame's speculative pass, on the whole grid (reverse list code)\n"
          "    const uint32_t nf = level < 0 ? 0u : __builtin_amdgcn_readfirstlane(tsi::level_split(a, level, nb));\n"
          "    const uint32_t bwave = b * 4u + w;\n"

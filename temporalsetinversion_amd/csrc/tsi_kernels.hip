@@ -176,7 +176,8 @@ __global__ void k_enqueue_root(KArgs a) { a.cnt->root_pending = 1; }
 
 struct V3 { float x, y, z; };
 TSI_DI V3 v3(float x, float y, float z) { return V3{x, y, z}; }
-TSI_DI float vlength(V3 a) { return sqrtf((a.x * a.x + a.y * a.y) + a.z * a.z); }
+// (a sum of squares: +0 or more, +inf or NaN, where sqrt_nonneg equals sqrtf bit for bit)
+TSI_DI float vlength(V3 a) { return sqrt_nonneg((a.x * a.x + a.y * a.y) + a.z * a.z); }
 TSI_DI float wsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 
 constexpr float kTrEps = 0.0001f;     // compute.wgsl:4140

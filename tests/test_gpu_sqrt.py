@@ -1,5 +1,5 @@
 """The interval square root (SQRT_I) against the compiler's correctly rounded sqrtf, for every
-binary32 in [-0, +inf]: sqrt_nonneg leaves out the lowering's last step (x itself for +-0 and
+binary32 in [-0, +inf] and every NaN: sqrt_nonneg leaves out the lowering's last step (x itself for +-0 and
 +inf), which is redundant on that domain, and this pins that claim on the hardware whose square
 root instruction the method corrects."""
 import ctypes as C
@@ -29,6 +29,10 @@ def test_sqrt_nonneg_equals_sqrtf_on_every_nonnegative_binary32():
     assert _check(0x00000000, 0x7F800001) == (0, 0xFFFFFFFF, 0x7F800001)
     # -0
     assert _check(0x80000000, 1) == (0, 0xFFFFFFFF, 1)
+    # NaNs of either sign (the trace's vector lengths take sqrt_nonneg too, whose argument is a
+    # sum of squares: >= +0, +inf or NaN)
+    assert _check(0x7F800001, 0x007FFFFF) == (0, 0xFFFFFFFF, 0x007FFFFF)
+    assert _check(0xFF800001, 0x007FFFFF) == (0, 0xFFFFFFFF, 0x007FFFFF)
 
 
 def test_sqrt_check_rejects_bad_ranges():   # argument checks come before any device call
