@@ -191,7 +191,67 @@ def sphere_unit(radius: float = 1.0) -> str:
     return T.to_ll(T.sub(T.length3(T.x, T.y, T.z), radius), "sphere")
 
 
+def mixed(seed: int = 0, n: int = 12) -> str:
+    """A seeded union/intersection/difference of n primitives built from most of the forms the
+    evaluators distinguish (the generator of the randomised parity tests): spheres and boxes,
+    planes with a literal on the left (c - x), products of two registers, divisions by
+    1 + q^2, MIN/MAX against literals, a square root of |u v|, and a sine of a value the primitive
+    uses twice (a transcendental record whose operand has two uses: the reverse pass cannot
+    short-circuit it).  Centres move with t."""
+    rng = np.random.default_rng(seed)
+    T = Tape()
+    d = None
+    for i in range(n):
+        c = rng.uniform(-1.0, 1.0, 3)
+        w = float(rng.uniform(0.5, 3.0))
+        ph = float(rng.uniform(0.0, 6.283))
+        amp = float(rng.uniform(0.0, 0.25))
+        # moving centre: c_x(t) = c0 + amp sin(w t + ph)
+        cx = T.add(T.mul(T.sin(T.add(T.mul(T.t, w), ph)), amp), float(c[0]))
+        q = [T.sub(T.x, cx), T.sub(T.y, float(c[1])), T.sub(T.z, float(c[2]))]
+        size = float(rng.uniform(0.1, 0.4))
+        kind = int(rng.integers(0, 7))
+        if kind == 0:        # sphere
+            prim = T.sub(T.length3(*q), size)
+        elif kind == 1:      # box
+            e = [T.sub(T.abs(q[j]), size * float(rng.uniform(0.5, 1.0))) for j in range(3)]
+            prim = T.max(T.max(e[0], e[1]), e[2])
+        elif kind == 2:      # slab between two planes: c - (a x + b y), then clamped
+            pl = T.add(T.mul(q[0], float(rng.uniform(0.2, 1.0))), T.mul(q[1], float(rng.uniform(-1.0, 1.0))))
+            prim = T.max(T.sub(size, pl), T.sub(pl, 2.0 * size))
+        elif kind == 3:      # saddle: q0 q1 - size, a product of two registers
+            prim = T.sub(T.mul(q[0], q[1]), T.mul(q[2], size))
+        elif kind == 4:      # bump: c / (1 + q0^2) against the height, min/max with literals
+            bump = T.div(size, T.add(T.sq(q[0]), 1.0))
+            prim = T.max(T.min(T.sub(q[1], bump), 0.5), -0.5)
+        elif kind == 5:      # sqrt(|q0 q2|) + |q1| - size
+            prim = T.sub(T.add(T.sqrt(T.abs(T.mul(q[0], q[2]))), T.abs(q[1])), size)
+        else:                # rippled sphere: |q| - size + 0.05 sin(s) with s used twice
+            s_ = T.mul(q[1], float(rng.uniform(2.0, 8.0)))
+            r = T.add(T.mul(T.sin(s_), 0.05), T.mul(s_, 0.01))
+            prim = T.add(T.sub(T.length3(*q), size), r)
+        if d is None:
+            d = prim
+        else:
+            op = int(rng.integers(0, 6))
+            if op < 3:
+                d = T.min(d, prim)
+            elif op == 3:
+                d = T.smin(d, prim, float(rng.uniform(0.02, 0.1)))
+            elif op == 4:
+                d = T.max(d, T.neg(prim))
+            else:
+                d = T.max(d, T.sub(prim, 0.6))
+    return T.to_ll(d, "mixed")
+
+
+def mixed_scenes() -> list:
+    """(seed, n) of the randomised parity tests' scenes (their kernels are compiled ahead by the build)."""
+    return [(11, 6), (12, 10), (13, 14), (14, 20)]
+
+
 SCENES = {
+    "mixed": mixed,
     "spheres": spheres,
     "spheres_linear": lambda seed=0: spheres(seed, linear=True),
     "capsules": capsules,
