@@ -547,9 +547,11 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             break;
         if ((rc = dmalloc(&c->dlists[0], c->dlist_cap)) || (rc = dmalloc(&c->dlists[1], c->dlist_cap)) ||
             (rc = dmalloc(&c->pending, c->nblocks)) || (rc = dmalloc(&c->free_stack, c->nblocks)) ||
-            (rc = dmalloc(&c->cnt, 1)) || (rc = dmalloc(&c->tslot[0].visits, 1)) ||
-            (rc = dmalloc(&c->tslot[1].visits, 1)))
+            (rc = dmalloc(&c->cnt, 1)))
             break;
+        for (auto &t : c->tslot)
+            if (!rc) rc = dmalloc(&t.visits, 1);
+        if (rc) break;
         if (hipMemsetAsync(c->cnt, 0, sizeof(Counters), c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
         // level kernels stride over their frontier; 4 blocks (16 wavefronts) per CU for both lists
         // (measured: the reverse list of a large frontier needs 4 wavefronts per SIMD to hide its
