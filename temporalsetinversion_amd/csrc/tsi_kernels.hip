@@ -681,8 +681,8 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
     const SpecLaunch *sp = fl.tape_eval ? fl.spec : nullptr;
     if (sp) {   // the speculative pass, beside the previous frame's last levels
         if (sp->wait) hipStreamWaitEvent(sp->stream, sp->wait, 0);
-        hipEventRecord(sp->begin, sp->stream);
         hipLaunchKernelGGL(k_spec_gather, dim3(1), dim3(1024), 0, sp->stream, sp->args);
+        hipEventRecord(sp->begin, sp->stream);   // times the tape kernel's launch only
         KArgs ka = sp->args;
         int level = -1;
         float t = fl.time;

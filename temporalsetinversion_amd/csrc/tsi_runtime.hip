@@ -501,11 +501,11 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
         bool evok = true;
         for (auto &e : c->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         for (auto &f : c->fslot) {
-            for (hipEvent_t *e : {&f.begin, &f.end, &f.spec0, &f.spec1})
+            for (hipEvent_t *e : {&f.begin, &f.end, &f.spec1})   // spec1: the frame's level 0 waits for it
                 evok = evok && hipEventCreate(e) == hipSuccess;
             // events that only time the levels (nothing waits on them): without the system-scope
             // fence, whose cache writeback and invalidation between two levels cost ~2 % of a frame
-            for (hipEvent_t *e : {&f.eval0, &f.eval1})
+            for (hipEvent_t *e : {&f.eval0, &f.eval1, &f.spec0})
                 evok = evok && hipEventCreateWithFlags(e, hipEventDisableSystemFence) == hipSuccess;
             for (hipEvent_t &e : f.level)
                 evok = evok && hipEventCreateWithFlags(&e, hipEventDisableSystemFence) == hipSuccess;
