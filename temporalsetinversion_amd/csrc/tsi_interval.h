@@ -251,6 +251,9 @@ TSI_DI float sqrt_nonneg(float x) {
     return tiny ? r * 0x1p-16f : r;
 }
 TSI_DI Iv SQRT_I(Iv a) { return I(sqrt_nonneg(wmax(a.lo, 0.0f)), sqrt_nonneg(wmax(a.hi, 0.0f))); }  // 2572
+// SQRT_I of an interval whose bounds are each +0, positive or +inf (never -0, negative or NaN),
+// where wmax(v, 0) is v: the compiled modules' form for operands they prove so
+TSI_DI Iv SQRT_NN(Iv a) { return I(sqrt_nonneg(a.lo), sqrt_nonneg(a.hi)); }
 TSI_DI Iv EXP_I(Iv a) { return I(det_exp(a.lo), det_exp(a.hi)); }                      // 2576
 TSI_DI Iv LOG_I(Iv a) { return I(det_log(wmax(a.lo, 0.0f)), det_log(wmax(a.hi, 0.0f))); }  // 2584
 TSI_DI Iv DIV_II(Iv a, Iv b) {                                                           // 2598

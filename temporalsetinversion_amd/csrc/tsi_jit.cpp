@@ -313,6 +313,28 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
     // records whose fix is the identity on the domain the kernels are specialised for
     const std::vector<char> ff = domain ? fix_free(tape, n, fwd, *domain) : std::vector<char>(n, 0);
     auto fixed = [&](uint32_t j, const std::string &e) { return ff[j] ? e : "fixk(" + e + ", K)"; };
+    // Unpruned modules on a domain: records whose value has both bounds +0, positive or +inf,
+    // never -0, negative or NaN (a square, an absolute value or a square root of a NaN-free
+    // operand, sums of such values and of non-negative literals; fix keeps the property).  A
+    // square root of such a value needs no clamp at 0 (SQRT_NN: the same bits as SQRT_I).
+    std::vector<char> pz(n, 0);
+    if (!prune && domain)
+        for (uint32_t j = 4; j < n; ++j) {
+            const uint32_t side = side_of(tape[j]), form = fwd[j - 4].form;
+            const uint32_t a = larg_of(tape[j]), b = rarg_of(tape[j]);
+            auto nan_free = [&](uint32_t k) { return k < 4u || ff[k]; };   // inputs: box coordinates, times
+            const float c = tape[j].literal;
+            if (form == F_SQR || form == F_ABS) pz[j] = side == 1 && nan_free(a);
+            else if (form == F_SQRT) pz[j] = side == 1 && pz[a];
+            else if (form == F_ADD_RR) pz[j] = side == 0 && pz[a] && pz[b];
+            else if (form == F_ADD_RL) pz[j] = side == 1 && pz[a] && c >= 0.0f && !std::signbit(c) && c <= kBig;
+        }
+    // the forward expression of record j over operand expressions l, r
+    auto fwd_expr = [&](uint32_t j, const std::string &l, const std::string &r) {
+        const uint32_t form = fwd[j - 4].form;
+        if (form == F_SQRT && side_of(tape[j]) == 1 && pz[larg_of(tape[j])]) return fixed(j, "SQRT_NN(" + l + ")");
+        return fixed(j, "fwd_form_t<true>(" + std::to_string(form) + "u, " + l + ", " + r + ", " + lit(tape[j].literal) + ")");
+    };
     // the variant being emitted: pruned (region mask, parent modes, choices) or full
     bool pr = prune;
     auto bit = [&](uint32_t r) {   // uniform test of region r's bit in the wave's region mask
@@ -544,7 +566,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
             const std::string w = std::to_string(m / 16), sh = std::to_string(2 * (m % 16)) + "u";
             o += "        cw" + w + " |= " + (is_min ? "min_choice(" : "max_choice(") + l + ", " + rhs + ") << " + sh + ";\n";
         } else {
-            o += "        " + V(j) + " = " + fixed(j, "fwd_form_t<true>(" + form + ", " + l + ", " + rr + ", " + c + ")") + ";\n";
+            o += "        " + V(j) + " = " + fwd_expr(j, l, rr) + ";\n";
         }
         if (slot[j] >= 0)
             o += "        if (PUSH) " + P(uint32_t(slot[j])) + " = make_float2(" + V(j) + ".lo, " + V(j) + ".hi);\n";
@@ -706,8 +728,7 @@ std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::v
                 const uint32_t side = side_of(tape[k]);
                 const std::string l = (side == 0 || side == 1) ? saved(j, int(larg_of(tape[k]))) : "ZI";
                 const std::string r = (side == 0 || side == 2) ? saved(j, int(rarg_of(tape[k]))) : "ZI";
-                v = fixed(uint32_t(k), "fwd_form_t<true>(" + std::to_string(fwd[k - 4].form) + "u, " + l + ", " + r +
-                                           ", " + lit(tape[k].literal) + ")");
+                v = fwd_expr(uint32_t(k), l, r);
             } else if (remat[k] >= 0) {   // recomputed from its saved register operand
                 const std::string x = saved(j, remat[k]);
                 const bool lr = side_of(tape[k]) == 2;
