@@ -195,7 +195,7 @@ class State:
         else:
             p.flags = 0 if prune else _lib.TSI_FLAG_NO_PRUNE
         # engine: "compiled" = the tape compiled into straight-line kernels at upload (pruned by
-        # regions of 16 records; unpruned modules use 32-record regions), "interpreter" =
+        # regions of 16 records; unpruned modules use 64-record regions), "interpreter" =
         # programs in constant memory, pruned per frontier group and record
         if engine not in ("compiled", "interpreter"):
             raise ValueError(f"engine must be 'compiled' or 'interpreter', not {engine!r}")

@@ -59,8 +59,9 @@ std::string lit(float c) {
 }
 
 // Records per region ($TSI_JIT_REGION, else 16 in pruned modules, whose region bits are the
-// pruning unit, and 32 in unpruned ones, where a longer region hides more of the next region's
-// load latency: +2-3 % measured).  Each region is its own basic block, entered through a uniform branch on
+// pruning unit, and 64 in unpruned ones, where a longer region hides more of the next region's
+// load latency and leaves the scheduler longer blocks: on the 1024^3 CSG frame 24, 32, 48, 64,
+// 96 and 128 records gave 381, 384-385, 386, 388-389, 389 and 391 M cell-evaluations/s).  Each region is its own basic block, entered through a uniform branch on
 // a value the compiler cannot see through (one scalar instruction), so that instruction
 // selection and scheduling stay local: without it the whole tape is one block in which every
 // load is hoisted to the top and every operation sunk to the bottom.  The reverse pass issues
@@ -71,7 +72,7 @@ uint32_t region_records(bool prune) {
         const long v = e ? std::strtol(e, nullptr, 10) : 0;
         return v >= 4 && v <= 1024 ? uint32_t(v) : 0u;
     }();
-    return g ? g : (prune ? 16u : 32u);
+    return g ? g : (prune ? 16u : 64u);
 }
 
 // Regions ahead whose saved-operand loads the reverse pass of an unpruned module has in flight
@@ -136,6 +137,16 @@ uint32_t level_prio() {
         return v >= 0 && v <= 3 ? uint32_t(v) : 3u;
     }();
     return p;
+}
+
+// Records per region of batch modules ($TSI_JIT_BATCH_REGION, default 32).
+uint32_t batch_region_records() {
+    static const uint32_t g = [] {
+        const char *e = std::getenv("TSI_JIT_BATCH_REGION");
+        const long v = e ? std::strtol(e, nullptr, 10) : 32;
+        return v >= 4 && v <= 1024 ? uint32_t(v) : 32u;
+    }();
+    return g;
 }
 
 // Batch modules with the skipped subtrees ($TSI_JIT_BATCH_SKIP=1; default without).
@@ -287,9 +298,9 @@ std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const st
     return free;
 }
 
-JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune) {
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune, uint32_t group) {
     JitLayout y;
-    y.group = region_records(prune);
+    y.group = group ? group : region_records(prune);
     y.regions = (L.nfwd + y.group - 1) / y.group;
     y.region_words = (y.regions + 31) / 32;
     for (uint32_t k = 0; k < L.nfwd; ++k) y.minmax += is_minmax_form(fwd[k].form) ? 1u : 0u;
@@ -300,10 +311,10 @@ JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prun
 
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
                        const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
-                       const JitDomain *domain, bool skip) {
+                       const JitDomain *domain, bool skip, uint32_t group) {
     std::string o;
     o.reserve(size_t(n) * 500u);
-    const JitLayout Y = jit_layout(fwd, L, prune);
+    const JitLayout Y = jit_layout(fwd, L, prune, group);
     const uint32_t G = Y.group, RW = Y.region_words, MW = Y.mode_words;
     auto region_of = [&](uint32_t j) { return (j - 4) / G; };
     // MIN/MAX index of record j (records in order)
@@ -1187,10 +1198,12 @@ std::string jit_source_batch(const std::vector<BatchTape> &tapes, const JitDomai
     for (size_t i = 0; i < tapes.size(); ++i) {
         const BatchTape &t = tapes[i];
         // without the skipped subtrees: the batched workload's wide swept windows leave most
-        // MIN/MAX undecided, and its 256 requests ran 6 % slower with them
+        // MIN/MAX undecided, and its 256 requests ran 6 % slower with them; 32-record regions
+        // (with 64: 611 against 643 frames/s batched, served p50 2.01 against 1.80 ms)
         const bool skip = batch_skip();
-        const std::string f = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, false, false, domain, skip);
-        const std::string r = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, true, false, domain, skip);
+        const uint32_t group = batch_region_records();
+        const std::string f = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, false, false, domain, skip, group);
+        const std::string r = jit_source(t.tape, t.n, *t.fwd, *t.rev, *t.low, true, false, domain, skip, group);
         o += "namespace tb" + std::to_string(i) + " {\nnamespace jf {   // forward-only list\n" + module_body(f);
         o += "}  // namespace jf\nnamespace jr {   // reverse list\n" + module_body(r);
         o += "}  // namespace jr\n}  // namespace tb" + std::to_string(i) + "\n";

@@ -16,7 +16,7 @@ struct JitLayout {
 };
 // (prune: the layout of pruned modules, the only ones that exchange records; unpruned modules
 // use longer regions)
-JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune);
+JitLayout jit_layout(const std::vector<FwdIns> &fwd, const Lowered &L, bool prune, uint32_t group = 0);
 
 // The domain a tape's kernels may be specialised for: every cell box inside `box` (lo xyz, hi
 // xyz), every t-window inside [t0, t1].  fix_free marks the records whose value no cell in it can
@@ -30,10 +30,11 @@ std::vector<char> fix_free(const tsi_gpu_instruction *tape, uint32_t n, const st
 // (tsi_tape_eval_fwd / tsi_tape_eval_rev), which jit_source_level replaces by one for both.  prune: regions dead for
 // all of a wavefront's parents are skipped and decided MIN/MAX become copies (liveness records
 // written for enqueued cells).  skip: the reverse pass of an unpruned module skips subtrees whose
-// sensitivity is [0, 0] in the whole wavefront.
+// sensitivity is [0, 0] in the whole wavefront.  group: records per region (0: the default of
+// pruned or unpruned modules).
 std::string jit_source(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
                        const std::vector<RevIns> &rev, const Lowered &L, bool rev_kernel, bool prune,
-                       const JitDomain *domain = nullptr, bool skip = true);
+                       const JitDomain *domain = nullptr, bool skip = true, uint32_t group = 0);
 // Both level kernels' code in one module with one kernel, tsi_tape_eval_level(KArgs, level,
 // time), that runs a level's forward-only and reverse lists side by side (blockDim 256).
 std::string jit_source_level(const tsi_gpu_instruction *tape, uint32_t n, const std::vector<FwdIns> &fwd,
