@@ -610,6 +610,29 @@ __global__ void __launch_bounds__(256) k_trace(KArgs a, TraceArgs t) {
 // and its wavefronts take the 8x8 quadrants, as k_trace's do (a persistent grid taking tiles
 // from a counter measured slower: a frame's slowest tiles then ran one after another in a few
 // wavefronts, where the hardware dispatcher starts each block on the first free slot).
+// The same pixels on a grid of a few blocks per CU: each wavefront claims 8x8-pixel quadrants
+// (row-major over the frame) from a counter until none is left.  A full launch of 16x16 blocks
+// keeps every CU's wave slots filled with trace wavefronts while any block is pending, so the
+// next frame's level kernels, launched beside it, wait for slots; this leaves the CUs most of
+// their slots.  Each pixel is shaded by the same code as in k_trace: the frame is the same.
+template <bool CIN>
+__global__ void __launch_bounds__(256) k_trace_quads(KArgs a, TraceArgs t, uint32_t *next_quad, uint32_t quads_x,
+                                                     uint32_t nquads) {
+    __shared__ uint32_t stack_lds[16 * 256];
+    const uint32_t l = threadIdx.x & 63u;
+    uint32_t visits = 0;
+    for (;;) {
+        uint32_t q = 0;
+        if (l == 0) q = atomicAdd(next_quad, 1u);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= nquads) break;
+        const uint32_t x = (q % quads_x) * 8u + (l & 7u), y = (q / quads_x) * 8u + (l >> 3);
+        if (x < t.width && y < t.height) shade_pixel<CIN>(a, t, x, y, stack_lds + threadIdx.x, visits);
+    }
+    uint64_t v = wave_sum64(visits);
+    if (l == 0 && v) atomicAdd((unsigned long long *)t.visits, (unsigned long long)v);
+}
+
 template <bool CIN>
 __global__ void __launch_bounds__(256) k_trace_batch(const TraceJob *jobs, uint32_t njobs) {
     __shared__ uint32_t stack_lds[16 * 256];
@@ -781,7 +804,16 @@ int launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s) {
+int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s, uint32_t blocks, uint32_t *next_quad) {
+    if (blocks && next_quad) {   // persistent: `blocks` blocks, *next_quad zeroed on the stream before
+        const uint32_t qx = (t.width + 7u) / 8u, nq = qx * ((t.height + 7u) / 8u);
+        const uint32_t nb = std::min(blocks, (nq + 3u) / 4u);
+        if (t.shading & TSI_SHADE_CINEMATIC)
+            hipLaunchKernelGGL(k_trace_quads<true>, dim3(nb), dim3(256), 0, s, a, t, next_quad, qx, nq);
+        else
+            hipLaunchKernelGGL(k_trace_quads<false>, dim3(nb), dim3(256), 0, s, a, t, next_quad, qx, nq);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     dim3 grid((t.width + 15u) / 16u, (t.height + 15u) / 16u);
     if (t.shading & TSI_SHADE_CINEMATIC) hipLaunchKernelGGL(k_trace<true>, grid, dim3(256), 0, s, a, t);
     else hipLaunchKernelGGL(k_trace<false>, grid, dim3(256), 0, s, a, t);

@@ -42,7 +42,9 @@ int launch_frame(const KArgs &a, const FrameLaunch &fl, hipStream_t s, hipEvent_
                  const hipEvent_t *e_level);
 int launch_reset(const KArgs &a, const float box[6], uint64_t nodes, hipStream_t s);
 int launch_enqueue_root(const KArgs &a, hipStream_t s);
-int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s);
+// blocks > 0 (with next_quad, a counter zeroed on `s` before): the persistent form on that many
+// blocks (k_trace_quads); else one 16x16-pixel block per tile (k_trace)
+int launch_trace(const KArgs &a, const TraceArgs &t, hipStream_t s, uint32_t blocks = 0, uint32_t *next_quad = nullptr);
 int launch_snapshot(const KArgs &a, float4 *s_st, float4 *s_blo, float4 *s_bhi, float4 *s_grad, uint64_t nodes,
                     uint32_t grid, hipStream_t s);
 int launch_rgba8(const float4 *in, uchar4 *out, uint32_t n, hipStream_t s);

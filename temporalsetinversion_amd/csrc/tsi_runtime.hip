@@ -91,6 +91,15 @@ struct tsi_ctx {
     // events around each level's launch (tsi_stats.eval_kernel_ms as the levels' summed kernel
     // time); $TSI_LEVEL_EVENTS=0 leaves them out (eval_kernel_ms is then the levels' span)
     bool level_events = true;
+    // The trace's form.  A full launch of 16x16-pixel blocks fills every CU's wave slots while
+    // blocks are pending, and the next frame's level kernels, launched beside it, wait for slots;
+    // the persistent form on one block per CU leaves them free but, with fewer wavefronts, takes
+    // ~15 % longer.  It is chosen while its trace time stays under 1.6 set inversions (the two
+    // trace slots then keep up), from the last frame's measured times; $TSI_TRACE_BLOCKS fixes
+    // it (0: full launches; n: the persistent form on n blocks per CU).
+    int trace_blocks_env = -1;
+    bool trace_persistent = false;
+    float last_si_ms = 0.0f;
     // the pass runs on its own stream with its own saved-operand stack (one block per CU), after
     // the previous frame's levels up to spec_max have run (spec_ready, recorded by that frame)
     hipStream_t spec_stream = nullptr;
@@ -109,6 +118,7 @@ struct tsi_ctx {
     uint32_t grid_fwd = 0, grid_rev = 0, grid_misc = 0;
     uint64_t last_visits = 0;
     double last_trace_ms = 0.0;
+    bool last_trace_persistent = false;
     uint64_t used_nodes = 8;            // nodes in use after the last collected set inversion
     double trace_ms_total = 0.0;
     uint64_t trace_visits_total = 0, trace_count = 0;
@@ -134,6 +144,7 @@ struct tsi_ctx {
         float *depth = nullptr;
         size_t px = 0;
         unsigned long long *visits = nullptr;
+        bool persistent = false;   // traced by k_trace_quads
     };
     static constexpr uint32_t kFrameSlots = 2, kTraceSlots = 2;
     FrameSlot fslot[kFrameSlots];
@@ -534,6 +545,10 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
                 if (!rc) rc = dmalloc(&x, c->list_cap);
         if (rc) break;
         if (const char *e = std::getenv("TSI_LEVEL_EVENTS")) c->level_events = std::strtol(e, nullptr, 10) != 0;
+        if (const char *e = std::getenv("TSI_TRACE_BLOCKS")) {
+            const long v = std::strtol(e, nullptr, 10);
+            c->trace_blocks_env = v >= 0 && v <= 16 ? int(v) : 0;
+        }
         // speculative pass: up to 2^16 listed parents (8 records of 64 B each: 32 MiB)
         c->spec_cap = 1u << 16;
         {
@@ -550,7 +565,7 @@ int tsi_create(const tsi_params *params, tsi_ctx **out) {
             (rc = dmalloc(&c->cnt, 1)))
             break;
         for (auto &t : c->tslot)
-            if (!rc) rc = dmalloc(&t.visits, 1);
+            if (!rc) rc = dmalloc(&t.visits, 2);   // the visit count, then the persistent trace's quad counter
         if (rc) break;
         if (hipMemsetAsync(c->cnt, 0, sizeof(Counters), c->stream) != hipSuccess) { rc = TSI_E_HIP; break; }
         // level kernels stride over their frontier; 4 blocks (16 wavefronts) per CU for both lists
@@ -755,6 +770,7 @@ int tsi_set_inversion_result(tsi_ctx *c, double *gpu_ms, tsi_stats *stats) {
     const Counters &h = *f.h;
     float ms = 0.0f, ems = 0.0f;
     hipEventElapsedTime(&ms, f.begin, f.end);
+    c->last_si_ms = ms;
     hipEventElapsedTime(&ems, f.eval0, f.eval1);
     if (gpu_ms) *gpu_ms = ms;
     if (stats) {
@@ -815,6 +831,7 @@ int harvest_trace(tsi_ctx *c) {
     hipEventElapsedTime(&tms, t.begin, t.end);
     c->last_visits = *t.visits_h;
     c->last_trace_ms = tms;
+    c->last_trace_persistent = t.persistent;
     c->trace_ms_total += tms;
     c->trace_visits_total += *t.visits_h;
     c->trace_count += 1;
@@ -876,9 +893,22 @@ int tsi_trace_async(tsi_ctx *c) {
     a.st = ts.st;
     a.blo = ts.blo;
     a.bhi = ts.bhi;
-    HIP_OK(hipMemsetAsync(ts.visits, 0, sizeof(unsigned long long), ts.stream));
+    HIP_OK(hipMemsetAsync(ts.visits, 0, 2 * sizeof(unsigned long long), ts.stream));
     HIP_OK(hipEventRecord(ts.begin, ts.stream));
-    if (launch_trace(a, t, ts.stream) != 0) return TSI_E_HIP;
+    uint32_t blocks = 0;
+    if (c->trace_blocks_env >= 0) {
+        blocks = uint32_t(c->trace_blocks_env) * uint32_t(c->num_cu);
+    } else if (c->last_si_ms > 0.0f && c->last_trace_ms > 0.0f) {
+        // the persistent form's expected time, from the last trace in either form (+/-10 %
+        // hysteresis around the switch)
+        const float tp = c->last_trace_persistent ? c->last_trace_ms : 1.15f * c->last_trace_ms;
+        const float limit = 1.6f * c->last_si_ms * (c->trace_persistent ? 1.1f : 0.9f);
+        c->trace_persistent = tp < limit;
+        blocks = c->trace_persistent ? uint32_t(c->num_cu) : 0u;
+    }
+    ts.persistent = blocks != 0;
+    if (launch_trace(a, t, ts.stream, blocks, reinterpret_cast<uint32_t *>(ts.visits + 1)) != 0)
+        return TSI_E_HIP;
     HIP_OK(hipMemcpyAsync(ts.visits_h, ts.visits, sizeof(unsigned long long), hipMemcpyDeviceToHost, ts.stream));
     HIP_OK(hipEventRecord(ts.end, ts.stream));
     c->last_tslot = k;
