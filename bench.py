@@ -173,7 +173,7 @@ def summarize_profile(args) -> None:
             lines.append(f"  level {d:>4} {sum(v) / len(v):>9.1f} {min(v):>9.1f} {max(v):>9.1f}")
     lines.append("")
     lines.append("timed frames only (the figures bench.py reads):")
-    for name in ("tsi_tape_eval_level", "void tsi::k_trace<false>"):
+    for name in ("tsi_tape_eval_level", "void tsi::k_trace<false>", "void tsi::k_trace_quads<false>"):
         if name in timed:
             n, t = timed[name]
             lines.append(f"{name} timed {n} dispatches avg_us {t / n:.2f}")
@@ -204,7 +204,7 @@ def summarize_profile(args) -> None:
         lines += ["", f"copy calibration ({args.copy_mib} MiB device-to-device): FETCH_SIZE scale {sf:.3f}, "
                       f"WRITE_SIZE scale {sw:.3f} (bytes = KiB x scale x 1024); level launches of the timed frames",
                   f"{'kernel':<24} {'':>4} {'dispatches':>10} {'fetch_KiB/disp':>15} {'write_KiB/disp':>15} {'bytes/disp':>15}"]
-        for k in ("tsi_tape_eval_level", "void tsi::k_trace<false>", "tsi::k_snapshot"):
+        for k in ("tsi_tape_eval_level", "void tsi::k_trace<false>", "void tsi::k_trace_quads<false>", "tsi::k_snapshot"):
             if k in f and k in w:
                 vf, vw = f[k], w[k]
                 if not vf or not vw:
@@ -967,7 +967,8 @@ def main():
                      "eval_ms_per_frame": tot["kernel_ms"] / args.steps,
                      "eval_span_ms_per_frame": tot["eval_ms"] / args.steps,
                      "ops_per_frame": ops / args.steps},
-        "roofline_trace": {"bound": "hbm", "kernel": "k_trace", "achieved": trace_gbs, "peak": PEAK_HBM_GBS,
+        "roofline_trace": {"bound": "hbm", "kernel": "k_trace / k_trace_quads (the form the runtime picks per frame)",
+                           "achieved": trace_gbs, "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": trace_gbs / PEAK_HBM_GBS,
                            "trace_ms_per_frame": trace["ms"] / args.steps,
                            "node_visits_per_frame": trace["visits"] / args.steps},
