@@ -118,3 +118,64 @@ def test_generated_modules_stay_inside_their_buffers(scene, prune):
                 depth += line.count("{") - line.count("}")
             if not prune:
                 assert set(first_load.values()) == {1}, f"{name}: a load inside a region block"
+
+
+# ---- the reverse pass's skipped subtrees (unpruned modules) ----
+
+# forms whose product with a zero sensitivity can be NaN (an infinite derivative bound): a record
+# of such a form that is not short-circuited (no z_J test: an operand with several uses) must
+# never sit inside a subtree the reverse pass may skip, or skipping would drop that NaN
+_UNSAFE_FORMS = {6, 7, 8, 9, 16, 18, 21, 22, 25, 26, 27, 32, 33, 34}
+
+
+def _guarded_rules(src):
+    """(record, form, guarded) of the rev_form_t rules of tape_reverse, guarded when the rule sits
+    in an `if (lv<g>) {` block (not in its else part)."""
+    body = _functions(src)["tape_reverse"][0]
+    out, stack, depth = [], [], 0
+    for line in body.split("\n"):
+        s = line.strip()
+        if s == "} else {" and stack and stack[-1][1] == depth:
+            stack[-1] = ("else", depth)
+        m = re.match(r"const Iv2 d_(\d+) = rev_form_t<true>\((\d+)u", s)
+        if m:
+            out.append((int(m.group(1)), int(m.group(2)), any(k == "guard" for k, _ in stack)))
+        if re.match(r"if \(lv\d+\) \{$", s):
+            stack.append(("guard", depth + 1))
+        depth += line.count("{") - line.count("}")
+        while stack and depth < stack[-1][1]:
+            stack.pop()
+    return out, body
+
+
+@pytest.mark.parametrize("scene", ["mixed-11", "mixed-12", "mixed-13", "mixed-14", "csg", "capsules"])
+def test_skipped_subtrees_hold_no_unsafe_rule(scene):
+    from temporalsetinversion_amd import scenes
+    if scene.startswith("mixed"):
+        seed = int(scene.split("-")[1])
+        text = scenes.mixed(seed, dict(scenes.mixed_scenes())[seed])
+    else:
+        text = scenes.SCENES[scene]()
+    src, _ = T.kernel_source(T.tape_from_ll(text), prune=False)
+    rules, body = _guarded_rules(src)
+    assert "if (lv" in body, "no skipped subtree generated"
+    assert any(g for _, _, g in rules), "no rule inside a skipped subtree"
+    short = set(int(x) for x in re.findall(r"const bool z_(\d+) =", body))
+    for j, form, guarded in rules:
+        if guarded and form in _UNSAFE_FORMS:
+            assert j in short, f"record {j} (form {form}) without a short-circuit inside a skipped subtree"
+
+
+def test_mixed_scenes_hold_unsafe_rules_outside_skipped_subtrees():
+    """The check above is not vacuous: the mixed scenes hold records of such forms without a
+    short-circuit (a sine whose operand has two uses), all outside the skipped subtrees."""
+    from temporalsetinversion_amd import scenes
+    found = 0
+    for seed, n in scenes.mixed_scenes():
+        src, _ = T.kernel_source(T.tape_from_ll(scenes.mixed(seed, n)), prune=False)
+        rules, body = _guarded_rules(src)
+        short = set(int(x) for x in re.findall(r"const bool z_(\d+) =", body))
+        unsafe = [(j, g) for j, form, g in rules if form in _UNSAFE_FORMS and j not in short]
+        assert not any(g for _, g in unsafe)
+        found += len(unsafe)
+    assert found > 0
